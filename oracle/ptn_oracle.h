@@ -1,0 +1,130 @@
+/*
+ * ptn_oracle.h -- FP64 CPU oracle for Partensor's dense CPD-ALS hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: tests/, the smoke
+ * entry point and the CPU-baseline leg of bench.py may load it; the product
+ * (the HIP library under partensor-toolbox_amd/) never links or calls it.
+ *
+ * It is a plain-C restatement of the reference's algorithm (Eigen is absent
+ * here, so the reference cannot be compiled).  Every function cites the
+ * reference file:line it follows.  Layouts follow the reference:
+ *   - tensors column-major, mode 0 fastest            (Tensor.hpp:63-64)
+ *   - factors I_n x R column-major (Eigen Matrix)       (Tensor.hpp:42)
+ *   - "packed" factor buffers = factor 0, factor 1, ... each I_n*R doubles.
+ * Built with -ffp-contract=off: every fused multiply-add is an explicit fma().
+ */
+#ifndef PTN_ORACLE_H
+#define PTN_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PTNO_MAX_ORDER 8
+
+/* Constraint values match partensor::Constraint (Constants.hpp:49-57). */
+enum { PTNO_UNCONSTRAINED = 0, PTNO_NONNEGATIVITY = 1, PTNO_ORTHOGONALITY = 2,
+       PTNO_SPARSITY = 3, PTNO_CONSTANT = 4 };
+
+/* MTTKRP forms. */
+enum { PTNO_MTTKRP_STANDARD = 0,  /* X_(n) * KR, Cpd.hpp:324 / MTTKRP.hpp:60-69      */
+       PTNO_MTTKRP_BLOCKED  = 1,  /* OpenMP blocked form, MTTKRP.hpp:102-225          */
+       PTNO_MTTKRP_DIMTREE  = 2 };/* dimension tree, DimTrees.hpp:668-928 (N = 3, 4) */
+
+/* Cost forms. */
+enum { PTNO_COST_EXPLICIT = 0,    /* ||X_(N) - A_N KR^T||, Cpd.hpp:145-149           */
+       PTNO_COST_GRAM     = 1 };  /* Gram form, CpdCUDA.hpp:137-142, CpdDimTree.hpp:156-160 */
+
+typedef struct {
+  int    max_iter;            /* PARTENSOR_basic.hpp:184-198 defaults: 20 */
+  double threshold_error;     /* 1e-3 */
+  double delta1, delta2;      /* Nesterov tolerances, 1e-2 */
+  int    acceleration;        /* 1 */
+  int    accel_coeff;         /* 3 */
+  int    accel_fail;          /* 0 */
+  int    normalization;       /* 1 */
+  int    constraints[PTNO_MAX_ORDER];
+  int    mttkrp_form;
+  int    cost_form;
+  int    threads;             /* OpenMP threads for the blocked form, 0 = runtime default */
+  int    max_inner;           /* Nesterov cap, NesterovMNLS.hpp:187 (50) */
+} ptno_options;
+
+typedef struct {
+  double frob_sq;     /* Status.frob_tns = ||X||^2 (Cpd.hpp:309) */
+  double f_value;     /* ||X - [[A]]||_F */
+  double rel_cost;    /* f_value / sqrt(frob_sq) */
+  int    ao_iter;
+  int    accel_coeff; /* options mutated by line_search_accel (Cpd.hpp:270-277) */
+  int    accel_fail;
+} ptno_status;
+
+/* One record per outer ALS iteration. */
+typedef struct {
+  double f_value;            /* after the cost of this sweep (before acceleration) */
+  double rel_cost;
+  double f_accel;            /* accelerated cost, 0 when not tried */
+  int    accel_tried;
+  int    accel_accepted;
+  int    nnls_iters[PTNO_MAX_ORDER];
+  int    zero_col_fallback[PTNO_MAX_ORDER];
+  double lipschitz[PTNO_MAX_ORDER];   /* L and mu of V_n before GLambda */
+  double mu[PTNO_MAX_ORDER];
+} ptno_sweep_log;
+
+void ptno_default_options(ptno_options *o);
+
+/* ---- seeded generator: Philox4x32-10 (Salmon et al., SC'11) ---- */
+void   ptno_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+double ptno_uniform(uint64_t seed, uint32_t stream, uint64_t idx);
+double ptno_normal(uint64_t seed, uint32_t stream, uint64_t idx);
+double ptno_exponential(uint64_t seed, uint32_t stream, uint64_t idx);
+/* dist: 0 = Uniform[0,1), 1 = Exponential(1).  Draw index = position in the packed buffer. */
+void   ptno_gen_factors(int N, const int64_t *dims, int R, uint64_t seed, int dist, double *packed);
+/* X = [[A]]   (DataGeneration.hpp:120-128 semantics) */
+void   ptno_cp_full(int N, const int64_t *dims, int R, const double *packed, double *X);
+/* X += s E, E_i ~ N(0,1) from (seed, stream 1), s = ||X||*10^(-snr/20)/||E||.  Returns s. */
+double ptno_add_noise(int64_t n, double *X, uint64_t seed, double snr_db);
+double ptno_square_norm(int64_t n, const double *X);          /* TensorOperations.hpp:144-149 */
+
+/* ---- layout ops ---- */
+void ptno_unfold(int N, const int64_t *dims, const double *X, int mode, double *out);          /* Matricization.hpp:101-124 */
+void ptno_partial_khatri_rao(int N, const int64_t *dims, int R, const double *packed, int mode, double *out); /* PartialKhatriRao.hpp:59-227 */
+
+/* ---- MTTKRP ---- */
+void ptno_mttkrp_naive(int N, const int64_t *dims, int R, const double *X, const double *packed, int mode, double *M);
+void ptno_mttkrp_blocked(int N, const int64_t *dims, int R, const double *X, const double *packed, int mode, int threads, double *M);
+/* Dimension-tree MTTKRPs of one sweep evaluated with a fixed set of factors (N = 3 or 4):
+   M_packed receives all N MTTKRPs (I_n x R each, col-major). */
+int  ptno_mttkrp_dimtree_all(int N, const int64_t *dims, int R, const double *X, const double *packed, double *M_packed);
+
+/* ---- small dense R x R pieces ---- */
+void   ptno_gram(int64_t I, int R, const double *A, double *G);                         /* Cpd.hpp:227 */
+void   ptno_partial_cwise(int N, int R, const double *grams_packed, int mode, double *V);/* PartialCwiseProd.hpp:54-222 */
+int    ptno_sym_extremes(int R, const double *V, double *L, double *mu);                 /* replaces ComputeSVD, NesterovMNLS.hpp:45-52 */
+int    ptno_sym_eig(int R, const double *V, double *evals, double *evecs);               /* full Jacobi (orthogonality branch) */
+void   ptno_glambda(double mu, double *L, double *lambda, double *q);                    /* NesterovMNLS.hpp:75-91 */
+double ptno_update_alpha(double alpha, double q);                                        /* NesterovMNLS.hpp:103-113 */
+int    ptno_nesterov_mnls(int64_t I, int R, const double *Z, const double *W, double d1, double d2,
+                          int max_inner, double *A, double *L_out, double *mu_out);    /* NesterovMNLS.hpp:181-234 */
+int    ptno_inverse(int R, const double *V, double *Vinv);                               /* Eigen .inverse() (PartialPivLU) */
+
+/* Normalize (Normalize.hpp:92-174), grams/factors packed. */
+void ptno_normalize(int N, const int64_t *dims, int R, int weight_factor, double *grams_packed, double *packed);
+
+/* ---- costs ---- */
+double ptno_cost_explicit(int N, const int64_t *dims, int R, const double *X, const double *packed);
+double ptno_cost_gram(int64_t I_last, int R, double frob_sq, const double *M_last, const double *A_last,
+                      const double *V_last, const double *G_last);
+
+/* ---- the ALS drivers: Cpd.hpp:288-356 / CpdOpenMP.hpp:266-329 / CpdDimTree.hpp:317-401 ---- */
+int ptno_cpd(int N, const int64_t *dims, int R, const double *X, const double *init_packed,
+             const ptno_options *opt, double *out_packed, ptno_status *st,
+             ptno_sweep_log *log, int log_cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
