@@ -1,0 +1,140 @@
+/*
+ * ptn_hip.h -- C ABI of the MI355X (gfx950) execution policy for Partensor's
+ * dense CPD-ALS.  Plain pointers and sizes only; no exceptions cross it.
+ *
+ * Layouts are the reference's (Tensor.hpp:42,63-64): tensors column-major with
+ * mode 0 fastest, factors I_n x R column-major, "packed" factor buffers =
+ * factor 0, factor 1, ... back to back.  Host buffers stay owned by the caller;
+ * a handle owns all device memory and streams and is used by one host thread
+ * at a time (handles are independent of each other).
+ */
+#ifndef PTN_HIP_H
+#define PTN_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PTN_MAX_ORDER 8
+#define PTN_MAX_RANK 64
+
+typedef struct ptn_handle_s *ptn_handle;
+
+typedef enum {
+  PTN_OK = 0,
+  PTN_ERR_INVALID = 1,      /* bad argument (dims, rank, pointer, option value) */
+  PTN_ERR_OOM = 2,          /* device allocation failed */
+  PTN_ERR_HIP = 3,          /* HIP runtime error (message in ptn_last_error) */
+  PTN_ERR_UNSUPPORTED = 4,  /* valid request this build does not implement */
+  PTN_ERR_STATE = 5         /* e.g. no tensor set on the handle */
+} ptn_status;
+
+/* Constraint values = partensor::Constraint (Constants.hpp:49-57). */
+enum { PTN_UNCONSTRAINED = 0, PTN_NONNEGATIVITY = 1, PTN_ORTHOGONALITY = 2, PTN_SPARSITY = 3, PTN_CONSTANT = 4 };
+/* Tensor storage on the device. */
+enum { PTN_TENSOR_F64 = 1, PTN_TENSOR_F32 = 2, PTN_TENSOR_BOTH = 3 };
+/* MTTKRP schedule: per-mode MTTKRP as cpd() (Cpd.hpp:324) or the dimension tree as
+   cpdDimTree() (CpdDimTree.hpp:355-366). */
+enum { PTN_MTTKRP_STANDARD = 0, PTN_MTTKRP_DIMTREE = 2 };
+/* Arithmetic of the tensor contractions. */
+enum { PTN_ARITH_F64 = 0,      /* FP64 tensor, FP64 matrix cores (the plain path)        */
+       PTN_ARITH_F32 = 1,      /* FP32 tensor, FP32 matrix cores, FP32 accumulate        */
+       PTN_ARITH_BF16X3 = 2 }; /* FP32 tensor, bf16x3 split products, FP32 accumulate    */
+
+/* Options<Tensor> fields the ALS path consumes (PARTENSOR_basic.hpp:283-336) plus
+   the execution choices of this policy. */
+typedef struct {
+  int      max_iter;            /* DefaultValues: 20   */
+  double   threshold_error;     /* 1e-3                */
+  double   nesterov_delta_1;    /* 1e-2                */
+  double   nesterov_delta_2;    /* 1e-2                */
+  int      accel_coeff;         /* 3                   */
+  int      accel_fail;          /* 0                   */
+  int      acceleration;        /* 1                   */
+  int      normalization;       /* 1                   */
+  int      constraints[PTN_MAX_ORDER];  /* nonnegativity */
+  /* policy choices */
+  int      mttkrp_form;         /* PTN_MTTKRP_STANDARD */
+  int      arith;               /* PTN_ARITH_F64       */
+  int      max_inner;           /* Nesterov cap, NesterovMNLS.hpp:187: 50 (<= 255) */
+  int      fuse_accel;          /* dimension tree + FP32: fold the acceleration pass into the next left-node pass */
+} ptn_options;
+
+/* Status<Tensor> (PARTENSOR_basic.hpp:556-595) minus the factors, which are written
+   to a caller buffer. */
+typedef struct {
+  double frob_sq;      /* Status.frob_tns = ||X||^2 */
+  double f_value;      /* ||X - [[A]]||_F           */
+  double rel_cost;     /* f_value / sqrt(frob_sq)   */
+  int    ao_iter;
+  int    accel_coeff;  /* options as mutated by the acceleration bookkeeping */
+  int    accel_fail;
+  double sweep_seconds;  /* wall time of the outer loop (device-synchronised) */
+} ptn_result;
+
+/* One record per outer iteration (the reference logs iter / fvalue / rel, Cpd.hpp:317). */
+typedef struct {
+  double f_value, rel_cost, f_accel;
+  int    accel_tried, accel_accepted;
+  int    nnls_iters[PTN_MAX_ORDER];
+  int    zero_col_fallback[PTN_MAX_ORDER];
+  double lipschitz[PTN_MAX_ORDER];
+  double mu[PTN_MAX_ORDER];
+} ptn_sweep_log;
+
+/* ---- handle ---- */
+ptn_status  ptn_create(ptn_handle *h, int device);
+void        ptn_destroy(ptn_handle h);
+const char *ptn_last_error(ptn_handle h);
+void        ptn_default_options(ptn_options *o);
+int         ptn_abi_version(void);
+
+/* ---- the tensor (uploaded once; replaces the per-block host->managed copies of
+        CUDAMTTKRP.hpp:157-163) ---- */
+/* X: column-major FP64 host array.  storage: PTN_TENSOR_F64 / _F32 / _BOTH. */
+ptn_status ptn_set_tensor(ptn_handle h, int N, const int64_t *dims, const double *X, int storage);
+/* Synthetic tensor on the device: X = [[A]] with A drawn from (seed, dist: 0 U[0,1),
+   1 Exp(1)) plus Gaussian noise at snr_db (no noise when snr_db is NaN). */
+ptn_status ptn_generate_tensor(ptn_handle h, int N, const int64_t *dims, int R_true, uint64_t seed, int dist,
+                               double snr_db, int storage);
+/* Factors drawn on the device with the same generator, returned packed column-major. */
+ptn_status ptn_generate_factors(ptn_handle h, int N, const int64_t *dims, int R, uint64_t seed, int dist,
+                                double *packed_out);
+ptn_status ptn_tensor_norm_sq(ptn_handle h, double *out);
+
+/* ---- the ALS driver: replaces cpd(execution::cuda, ...) (Cpd.hpp:1438-1467 ->
+        CpdCUDA.hpp:277-383) and cpdDimTree(...) (CpdDimTree.hpp:807-825 -> 317-401)
+        for a tensor set on the handle and caller-given initial factors
+        (cpd(tnsX, R, options, factorsInit), Cpd.hpp:550-577) ---- */
+ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_options *opt,
+                   double *out_packed, ptn_result *res, ptn_sweep_log *log, int log_cap);
+
+/* ---- single operations (replace the reference functions named) ---- */
+/* mttkrp(), MTTKRP.hpp:60-69 / hybrid_mttkrp, CUDAMTTKRP.hpp:224-338: M (I_mode x R col-major). */
+ptn_status ptn_mttkrp(ptn_handle h, int R, const double *factors_packed, int mode, int arith, double *M_out);
+/* the N MTTKRPs of one dimension-tree sweep with fixed factors (DimTrees.hpp:668-928), packed. */
+ptn_status ptn_mttkrp_dimtree(ptn_handle h, int R, const double *factors_packed, int arith, double *M_packed_out);
+/* NesterovMNLS, NesterovMNLS.hpp:181-234: Z (R x R), W, A (I x R col-major); A in/out. */
+ptn_status ptn_nesterov_mnls(ptn_handle h, int64_t I, int R, const double *Z, const double *W, double delta_1,
+                             double delta_2, int max_inner, double *A, int *iters, double *L, double *mu);
+/* ComputeSVD extremes for the symmetric V (NesterovMNLS.hpp:45-52). */
+ptn_status ptn_sym_extremes(ptn_handle h, int R, const double *V, double *L, double *mu);
+/* Normalize, Normalize.hpp:92-174: grams (N x R x R, col-major each) and factors in/out. */
+ptn_status ptn_normalize(ptn_handle h, int N, const int64_t *dims, int R, int weight_factor, double *grams_packed,
+                         double *factors_packed);
+/* ||X - [[A]]||_F for the handle's tensor (Cpd.hpp:131-135). */
+ptn_status ptn_cost_explicit(ptn_handle h, int R, const double *factors_packed, double *f);
+
+/* Average device time (ms) of `iters` back-to-back launches of one contraction
+   kernel on the handle's stream, bracketed by HIP events on that stream.
+   kernel: 0 = left node X x_{N-1} A_{N-1} (gemm_cm), 1 = right node X x_0 A_0 (gemm_ck),
+   2 = mode-0 standard MTTKRP.  bytes_out: algorithmic bytes of one launch. */
+ptn_status ptn_time_contraction(ptn_handle h, int kernel, int R, const double *factors_packed, int arith, int iters,
+                                double *avg_ms, double *bytes_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,316 @@
+// MTTKRP building blocks on FP64 matrix cores (v_mfma_f64_16x16x4_f64).
+//
+// The reference forms M_n = X_(n) * KR explicitly with Eigen GEMM per block
+// (MTTKRP.hpp:60-69, :195-220) or streams blocks host->device for cublasDgemm
+// (CUDAMTTKRP.hpp:157-175).  Here X stays resident and is read once per
+// contraction; Khatri-Rao rows are formed in registers from factor rows; the
+// unfolding X_(n) is index arithmetic, never a copy (Matricization.hpp:101-124).
+//
+// Two GEMM shapes cover every mode of every tree:
+//   gemm_cm:  out[m][r] = sum_k X[m + M*k] W(k)[r]   (contracted index strided;
+//             mode-0 MTTKRP, the dimension tree's X x_{N-1} A_{N-1})
+//   gemm_ck:  T[c][r]   = sum_a X[a + L*c] W(a)[r]   (contracted index contiguous;
+//             X x_0 A_0, the dimension tree's right node)
+// and contract_rows finishes a contraction against Khatri-Rao rows (the TTVs of
+// TensorPartialProduct_R, TensorOperations.hpp:196-213).
+//
+// MFMA 16x16x4 f64 operand layout (lane l, g = l/16, f = l%16):
+//   A (16x4): A[f][g]   B (4x16): B[g][f]   D (16x16): D[g+4v][f], v = 0..3
+//   (the D row layout was checked on gfx950 with single-entry tensors)
+#include "ptn_internal.hpp"
+
+namespace ptn {
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+struct KrArgs {
+  int nm;
+  uint32_t dim[kMaxOrder];
+  const double *fac[kMaxOrder];
+};
+
+KrArgs to_args(const KrSpec &w) {
+  KrArgs a{};
+  a.nm = w.nm;
+  for (int j = 0; j < w.nm; ++j) { a.dim[j] = (uint32_t)w.dim[j]; a.fac[j] = w.fac[j]; }
+  return a;
+}
+
+template <typename TX>
+__device__ __forceinline__ double ld(const TX *p) { return (double)__builtin_nontemporal_load(p); }
+
+// ---------------------------------------------------------------------------
+// gemm_cm: a block owns 64 consecutive rows and one k-chunk; its 4 waves take
+// interleaved k-steps of 4 columns and are summed through LDS at the end.
+// ---------------------------------------------------------------------------
+template <typename TX, int NT>
+__global__ __launch_bounds__(256) void k_gemm_cm_f64(const TX *__restrict__ X, int64_t Mrows, int64_t Ktot,
+                                                     KrArgs w, int ldr, int64_t kchunk, double *__restrict__ out) {
+  constexpr int RP = NT * 16;
+  __shared__ double red[64 * RP];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, f = lane & 15;
+  const int64_t mb = (int64_t)blockIdx.x * 64;
+  const int64_t kbeg = (int64_t)blockIdx.y * kchunk;
+  const int64_t kend = kbeg + kchunk < Ktot ? kbeg + kchunk : Ktot;
+
+  f64x4 acc[4][NT];
+#pragma unroll
+  for (int s = 0; s < 4; ++s)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[s][t] = f64x4{0.0, 0.0, 0.0, 0.0};
+
+  bool rv[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) rv[s] = mb + 16 * s + f < Mrows;
+
+  // digits of this lane's first column kk = kbeg + 4*wave + g (lowest mode fastest)
+  int64_t kk = kbeg + 4 * wave + g;
+  uint32_t dig[kMaxOrder];
+  {
+    uint64_t rem = (uint64_t)kk;
+    for (int j = 0; j < w.nm; ++j) { dig[j] = (uint32_t)(rem % w.dim[j]); rem /= w.dim[j]; }
+  }
+  for (; kk - g < kend; kk += 16) {
+    const bool kv = kk < kend;
+    double a[4];
+    const TX *xp = X + mb + f + Mrows * kk;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) a[s] = (kv && rv[s]) ? ld(xp + 16 * s) : 0.0;
+    double b[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      double p = 0.0;
+      if (kv) {
+        p = w.fac[0][(int64_t)dig[0] * ldr + 16 * t + f];
+        for (int j = 1; j < w.nm; ++j) p *= w.fac[j][(int64_t)dig[j] * ldr + 16 * t + f];
+      }
+      b[t] = p;
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int t = 0; t < NT; ++t) acc[s][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b[t], acc[s][t], 0, 0, 0);
+    // advance the odometer by 16 columns
+    dig[0] += 16;
+    for (int j = 0; j + 1 < w.nm && dig[j] >= w.dim[j]; ++j) {
+      uint32_t q = dig[j] / w.dim[j];
+      dig[j] -= q * w.dim[j];
+      dig[j + 1] += q;
+    }
+  }
+
+  // sum the 4 waves' tiles in LDS, in wave order
+  for (int src = 0; src < 4; ++src) {
+    if (wave == src) {
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            int row = 16 * s + g + 4 * v, col = 16 * t + f;
+            red[row * RP + col] = (src == 0 ? 0.0 : red[row * RP + col]) + acc[s][t][v];
+          }
+    }
+    __syncthreads();
+  }
+  double *o = out + ((int64_t)blockIdx.y * Mrows) * ldr;
+  for (int e = threadIdx.x; e < 64 * RP; e += 256) {
+    int row = e / RP, col = e % RP;
+    if (mb + row < Mrows) o[(mb + row) * ldr + col] = red[e];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// gemm_ck: a wave owns 64 fibers (4 MFMA row tiles) and the whole contraction.
+// Within a 16-wide k-block, lane group g supplies a = kb + 4g + t at MFMA step t,
+// so each lane reads 4 consecutive elements of its fiber.
+// ---------------------------------------------------------------------------
+template <typename TX, int NT>
+__global__ __launch_bounds__(256) void k_gemm_ck_f64(const TX *__restrict__ X, int64_t L, int64_t C, KrArgs w,
+                                                     int ldr, double *__restrict__ T) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, f = lane & 15;
+  const int64_t cb = ((int64_t)blockIdx.x * 4 + wave) * 64;
+  if (cb >= C) return;
+
+  f64x4 acc[4][NT];
+#pragma unroll
+  for (int s = 0; s < 4; ++s)
+#pragma unroll
+    for (int u = 0; u < NT; ++u) acc[s][u] = f64x4{0.0, 0.0, 0.0, 0.0};
+
+  bool cv[4];
+  const TX *xp[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    int64_t c = cb + 16 * s + f;
+    cv[s] = c < C;
+    xp[s] = X + L * (cv[s] ? c : 0);
+  }
+  for (int64_t kb = 0; kb < L; kb += 16) {
+    double x[4][4];
+    double b[4][NT];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int64_t a = kb + 4 * g + t;
+      const bool av = a < L;
+#pragma unroll
+      for (int s = 0; s < 4; ++s) x[s][t] = (av && cv[s]) ? ld(xp[s] + a) : 0.0;
+      // Khatri-Rao row of the contracted modes at a (lowest mode fastest)
+      const double *brow[kMaxOrder];
+      if (av) {
+        uint32_t rem = (uint32_t)a;
+        for (int j = 0; j < w.nm; ++j) {
+          uint32_t d = rem % w.dim[j];
+          rem /= w.dim[j];
+          brow[j] = w.fac[j] + (int64_t)d * ldr;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < NT; ++u) {
+        double p = 0.0;
+        if (av) {
+          p = brow[0][16 * u + f];
+          for (int j = 1; j < w.nm; ++j) p *= brow[j][16 * u + f];
+        }
+        b[t][u] = p;
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int u = 0; u < NT; ++u)
+          acc[s][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[s][t], b[t][u], acc[s][u], 0, 0, 0);
+  }
+#pragma unroll
+  for (int s = 0; s < 4; ++s)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      int64_t c = cb + 16 * s + g + 4 * v;
+      if (c < C) {
+#pragma unroll
+        for (int u = 0; u < NT; ++u) T[c * ldr + 16 * u + f] = acc[s][u][v];
+      }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// contract_rows: block (i, split) sums, for output index i, the rows of T whose
+// remaining digits fall in the split, weighted by the other modes' factor rows.
+// Threads: r = tid % RP, group = tid / RP; groups are summed in LDS in order.
+// ---------------------------------------------------------------------------
+struct RowSpec {
+  int nm, outmode;
+  uint32_t dim[kMaxOrder];
+  int64_t stride[kMaxOrder];   // row stride of each digit in T
+  const double *fac[kMaxOrder];
+};
+
+template <typename TT>
+__global__ __launch_bounds__(256) void k_contract_rows(const TT *__restrict__ T, int64_t ldt, RowSpec sp, int RP,
+                                                       int ldr, int64_t cchunk, int64_t Cother,
+                                                       double *__restrict__ partials) {
+  __shared__ double red[256];
+  const int r = threadIdx.x % RP, grp = threadIdx.x / RP, ngrp = 256 / RP;
+  const uint32_t i = blockIdx.x;
+  const int64_t c0 = (int64_t)blockIdx.y * cchunk;
+  const int64_t c1 = c0 + cchunk < Cother ? c0 + cchunk : Cother;
+  double acc = 0.0;
+  if (grp < ngrp) {
+    for (int64_t c = c0 + grp; c < c1; c += ngrp) {
+      uint64_t rem = (uint64_t)c;
+      int64_t row = (int64_t)i * sp.stride[sp.outmode];
+      double wv = 1.0;
+      for (int j = 0; j < sp.nm; ++j) {
+        if (j == sp.outmode) continue;
+        uint32_t d = (uint32_t)(rem % sp.dim[j]);
+        rem /= sp.dim[j];
+        row += (int64_t)d * sp.stride[j];
+        wv *= sp.fac[j][(int64_t)d * ldr + r];
+      }
+      acc = fma((double)T[row * ldt + r], wv, acc);
+    }
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  if (threadIdx.x < RP) {
+    double s = 0.0;
+    for (int q = 0; q < ngrp; ++q) s += red[q * RP + threadIdx.x];
+    partials[((int64_t)blockIdx.y * gridDim.x + i) * ldr + threadIdx.x] = s;
+  }
+}
+
+__global__ void k_reduce_splits(const double *__restrict__ partials, int splits, int64_t n, double *__restrict__ out) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int q = 0; q < splits; ++q) s += partials[q * n + e];
+    out[e] = s;
+  }
+}
+
+}  // namespace
+
+void gemm_cm_f64(hipStream_t s, const double *x64, const float *x32, int64_t Mrows, int64_t Ktot, const KrSpec &w,
+                 int ldr, int splits, double *out) {
+  KrArgs a = to_args(w);
+  int64_t kchunk = (Ktot + splits - 1) / splits;
+  dim3 grid((unsigned)((Mrows + 63) / 64), (unsigned)splits);
+  const int NT = ldr / 16;
+#define PTN_CM(NTV)                                                                              \
+  if (NT == NTV) {                                                                               \
+    if (x64) k_gemm_cm_f64<double, NTV><<<grid, 256, 0, s>>>(x64, Mrows, Ktot, a, ldr, kchunk, out); \
+    else     k_gemm_cm_f64<float, NTV><<<grid, 256, 0, s>>>(x32, Mrows, Ktot, a, ldr, kchunk, out);  \
+    return;                                                                                      \
+  }
+  PTN_CM(1) PTN_CM(2) PTN_CM(3) PTN_CM(4)
+#undef PTN_CM
+}
+
+void gemm_ck_f64(hipStream_t s, const double *x64, const float *x32, int64_t L, int64_t C, const KrSpec &w, int ldr,
+                 double *T) {
+  KrArgs a = to_args(w);
+  dim3 grid((unsigned)((C + 255) / 256));
+  const int NT = ldr / 16;
+#define PTN_CK(NTV)                                                                    \
+  if (NT == NTV) {                                                                     \
+    if (x64) k_gemm_ck_f64<double, NTV><<<grid, 256, 0, s>>>(x64, L, C, a, ldr, T);   \
+    else     k_gemm_ck_f64<float, NTV><<<grid, 256, 0, s>>>(x32, L, C, a, ldr, T);    \
+    return;                                                                            \
+  }
+  PTN_CK(1) PTN_CK(2) PTN_CK(3) PTN_CK(4)
+#undef PTN_CK
+}
+
+void contract_rows(hipStream_t s, const double *T64, const float *T32, int64_t ldt, const KrSpec &spec, int outmode,
+                   int ldr, int splits, double *partials) {
+  RowSpec sp{};
+  sp.nm = spec.nm;
+  sp.outmode = outmode;
+  int64_t st = 1, Cother = 1;
+  for (int j = 0; j < spec.nm; ++j) {
+    sp.dim[j] = (uint32_t)spec.dim[j];
+    sp.stride[j] = st;
+    st *= spec.dim[j];
+    sp.fac[j] = spec.fac[j];
+    if (j != outmode) Cother *= spec.dim[j];
+  }
+  int RP = ldr;
+  int64_t cchunk = (Cother + splits - 1) / splits;
+  dim3 grid((unsigned)spec.dim[outmode], (unsigned)splits);
+  if (T64) k_contract_rows<double><<<grid, 256, 0, s>>>(T64, ldt, sp, RP, ldr, cchunk, Cother, partials);
+  else     k_contract_rows<float><<<grid, 256, 0, s>>>(T32, ldt, sp, RP, ldr, cchunk, Cother, partials);
+}
+
+void reduce_splits(hipStream_t s, const double *partials, int splits, int64_t n, double *out) {
+  int64_t g = (n + 255) / 256;
+  if (g > 4096) g = 4096;
+  k_reduce_splits<<<(unsigned)g, 256, 0, s>>>(partials, splits, n, out);
+}
+
+}  // namespace ptn

@@ -1,0 +1,840 @@
+// C ABI of the MI355X execution policy: handle, tensor upload / generation, the
+// ALS driver (cpd() / cpdDimTree() semantics) and single-operation entry points.
+// Host control flow follows CPD::als (Cpd.hpp:288-356), CPD<T,cuda>::als
+// (CpdCUDA.hpp:277-383) and CPD_DIMTREE::als (CpdDimTree.hpp:317-401); every
+// tensor-sized and R x R computation runs in the HIP kernels.
+#include "../../include/ptn_hip.h"
+#include "ptn_internal.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <functional>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace ptn;
+
+struct ptn_handle_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int N = 0;
+  int64_t dims[kMaxOrder] = {0};
+  int64_t total = 0;
+  double *x64 = nullptr;
+  float *x32 = nullptr;
+  double *scal = nullptr;    // device scalars: [0] ||X||^2, [1..] scratch
+  double *part = nullptr;    // reduction partials for norms
+  bool frob_valid = false;
+  double frob_host = 0.0;
+};
+
+namespace {
+
+constexpr int kNormParts = 1024;
+
+struct Fail {
+  ptn_status code;
+};
+
+#define HIPCHK(expr)                                                                      \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess) {                                                               \
+      h->err = std::string(#expr) + ": " + hipGetErrorString(e_);                        \
+      throw Fail{e_ == hipErrorOutOfMemory ? PTN_ERR_OOM : PTN_ERR_HIP};                  \
+    }                                                                                     \
+  } while (0)
+
+#define REQUIRE(cond, code, msg)        \
+  do {                                  \
+    if (!(cond)) {                      \
+      h->err = (msg);                   \
+      throw Fail{code};                 \
+    }                                   \
+  } while (0)
+
+template <typename T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  ptn_handle h = nullptr;
+  DevBuf() = default;
+  DevBuf(ptn_handle hh, size_t count) { alloc(hh, count); }
+  void alloc(ptn_handle hh, size_t count) {
+    h = hh;
+    n = count ? count : 1;
+    HIPCHK(hipMalloc((void **)&p, n * sizeof(T)));
+    HIPCHK(hipMemsetAsync(p, 0, n * sizeof(T), h->stream));
+  }
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  operator T *() const { return p; }
+};
+
+ptn_status run(ptn_handle h, const std::function<void()> &fn);
+
+int round_up16(int R) { return (R + 15) / 16 * 16; }
+
+int64_t prod(const int64_t *d, int n) {
+  int64_t p = 1;
+  for (int k = 0; k < n; ++k) p *= d[k];
+  return p;
+}
+
+void check_dims(ptn_handle h, int N, const int64_t *dims) {
+  REQUIRE(N >= 3 && N <= kMaxOrder, PTN_ERR_INVALID, "tensor order must be in [3, 8]");
+  REQUIRE(dims != nullptr, PTN_ERR_INVALID, "dims is null");
+  for (int k = 0; k < N; ++k) REQUIRE(dims[k] >= 1 && dims[k] < (1ll << 31), PTN_ERR_INVALID, "bad dimension");
+  REQUIRE(prod(dims, N) / dims[0] < (1ll << 32), PTN_ERR_UNSUPPORTED, "prod of dims 1..N-1 must be < 2^32");
+}
+
+// ------------------------------------------------------------------------------------
+// working set of one decomposition
+// ------------------------------------------------------------------------------------
+struct Work {
+  ptn_handle h;
+  int N, R, ldr;
+  int64_t dims[kMaxOrder];
+  int64_t total, maxI, fsz;
+  int64_t off[kMaxOrder];
+  int arith;
+  DevBuf<double> fac, old, nrm, acc, grams, agr, V, M, Aold, colsum, part, T, YL, gpart, dpart, nscr, scal;
+  DevBuf<NnlsParams> prm;
+  DevBuf<uint64_t> mask;
+  int gparts = 64, dparts = 256;
+  int64_t part_cap = 0;
+
+  Work(ptn_handle hh, int R_, int arith_, bool need_tree) : h(hh), N(hh->N), R(R_), ldr(round_up16(R_)), arith(arith_) {
+    for (int k = 0; k < N; ++k) dims[k] = h->dims[k];
+    total = h->total;
+    maxI = 0;
+    int64_t o = 0;
+    for (int k = 0; k < N; ++k) {
+      off[k] = o;
+      o += dims[k] * ldr;
+      if (dims[k] > maxI) maxI = dims[k];
+    }
+    fsz = o;
+    fac.alloc(h, fsz); old.alloc(h, fsz); nrm.alloc(h, fsz); acc.alloc(h, fsz);
+    grams.alloc(h, (size_t)N * R * R); agr.alloc(h, (size_t)N * R * R); V.alloc(h, (size_t)R * R);
+    M.alloc(h, maxI * ldr); Aold.alloc(h, maxI * ldr); colsum.alloc(h, kMaxRank);
+    prm.alloc(h, kMaxOrder); mask.alloc(h, 4);
+    gpart.alloc(h, (size_t)gparts * R * R); dpart.alloc(h, dparts);
+    nscr.alloc(h, 2 * kMaxOrder * kMaxRank); scal.alloc(h, 8);
+    // partials: mode-0 split-K (<= 256 splits of I_0 x ldr) and row contractions
+    part_cap = std::max<int64_t>(256 * dims[0] * ldr, 64 * maxI * ldr);
+    if (need_tree && N == 4) part_cap = std::max<int64_t>(part_cap, 64 * dims[0] * dims[1] * ldr);
+    part.alloc(h, part_cap);
+    int64_t C = total / dims[0];
+    T.alloc(h, C * ldr);
+    if (need_tree) {
+      int64_t yl = (N == 3) ? dims[0] * dims[1] : dims[0] * dims[1];
+      YL.alloc(h, yl * ldr);
+    }
+  }
+  double *F(int k) { return fac.p + off[k]; }
+  double *Fb(double *base, int k) { return base + off[k]; }
+  double *G(int k) { return grams.p + (int64_t)k * R * R; }
+  const double *x64() const { return arith == PTN_ARITH_F64 ? h->x64 : nullptr; }
+  const float *x32() const { return arith == PTN_ARITH_F64 ? nullptr : h->x32; }
+};
+
+int pick_splits(int64_t blocks_per_split, int64_t work_per_split_unit, int64_t max_split) {
+  int64_t s = (2048 + blocks_per_split - 1) / blocks_per_split;
+  if (s > max_split) s = max_split;
+  if (s > work_per_split_unit) s = work_per_split_unit;
+  if (s < 1) s = 1;
+  return (int)s;
+}
+
+// M (row-major I_mode x ldr) = X_(mode) * KR(base factors), standard form.
+void mttkrp_standard(Work &w, double *base, int mode, double *M) {
+  ptn_handle h = w.h;
+  hipStream_t s = h->stream;
+  const int N = w.N;
+  const int64_t I0 = w.dims[0];
+  if (mode == 0) {
+    KrSpec k{};
+    k.nm = N - 1;
+    for (int j = 1; j < N; ++j) { k.dim[j - 1] = w.dims[j]; k.fac[j - 1] = w.Fb(base, j); }
+    int64_t Ktot = w.total / I0;
+    int64_t rowblocks = (I0 + 63) / 64;
+    int S = pick_splits(rowblocks, Ktot / 64, 256);
+    gemm_cm_f64(s, w.x64(), w.x32(), I0, Ktot, k, w.ldr, S, w.part);
+    reduce_splits(s, w.part, S, I0 * w.ldr, M);
+    return;
+  }
+  KrSpec k0{};
+  k0.nm = 1;
+  k0.dim[0] = I0;
+  k0.fac[0] = w.Fb(base, 0);
+  int64_t C = w.total / I0;
+  gemm_ck_f64(s, w.x64(), w.x32(), I0, C, k0, w.ldr, w.T);
+  KrSpec sp{};
+  sp.nm = N - 1;
+  for (int j = 1; j < N; ++j) { sp.dim[j - 1] = w.dims[j]; sp.fac[j - 1] = w.Fb(base, j); }
+  int64_t In = w.dims[mode], Cother = C / In;
+  int S = pick_splits(In, Cother / 64, 64);
+  contract_rows(s, w.T, nullptr, w.ldr, sp, mode - 1, w.ldr, S, w.part);
+  reduce_splits(s, w.part, S, In * w.ldr, M);
+}
+
+// Dimension-tree MTTKRP of `mode` (tree of ExprTree::Create, DimTrees.hpp:541-628),
+// recomputing the node it needs exactly where UpdateTree (DimTrees.hpp:862-928) would
+// in an ALS sweep: the left node at mode 0, the right node at mode N/2+... (first right leaf).
+void mttkrp_tree(Work &w, double *base, int mode, double *M) {
+  ptn_handle h = w.h;
+  hipStream_t s = h->stream;
+  const int N = w.N;
+  const int64_t *d = w.dims;
+  if (N == 3) {
+    if (mode == 0) {                 // left node {0,1}: X x_2 A_2
+      KrSpec k{};
+      k.nm = 1; k.dim[0] = d[2]; k.fac[0] = w.Fb(base, 2);
+      gemm_cm_f64(s, w.x64(), w.x32(), d[0] * d[1], d[2], k, w.ldr, 1, w.YL);
+    }
+    if (mode <= 1) {                 // leaves 0, 1: TTV of the left node
+      KrSpec sp{};
+      sp.nm = 2; sp.dim[0] = d[0]; sp.dim[1] = d[1]; sp.fac[0] = w.Fb(base, 0); sp.fac[1] = w.Fb(base, 1);
+      int S = pick_splits(d[mode], d[1 - mode] / 64, 64);
+      contract_rows(s, w.YL, nullptr, w.ldr, sp, mode, w.ldr, S, w.part);
+      reduce_splits(s, w.part, S, d[mode] * w.ldr, M);
+      return;
+    }
+    // right node {2}: (X x_0 A_0) TTV A_1
+    KrSpec k0{};
+    k0.nm = 1; k0.dim[0] = d[0]; k0.fac[0] = w.Fb(base, 0);
+    gemm_ck_f64(s, w.x64(), w.x32(), d[0], d[1] * d[2], k0, w.ldr, w.T);
+    KrSpec sp{};
+    sp.nm = 2; sp.dim[0] = d[1]; sp.dim[1] = d[2]; sp.fac[0] = w.Fb(base, 1); sp.fac[1] = w.Fb(base, 2);
+    int S = pick_splits(d[2], d[1] / 64, 64);
+    contract_rows(s, w.T, nullptr, w.ldr, sp, 1, w.ldr, S, w.part);
+    reduce_splits(s, w.part, S, d[2] * w.ldr, M);
+    return;
+  }
+  // N == 4: left {0,1} = X x_{2,3} (A_3 (.) A_2) fused; right {2,3} = X x_{0,1} (A_1 (.) A_0) fused
+  if (mode == 0) {
+    KrSpec k{};
+    k.nm = 2; k.dim[0] = d[2]; k.dim[1] = d[3]; k.fac[0] = w.Fb(base, 2); k.fac[1] = w.Fb(base, 3);
+    int64_t Mrows = d[0] * d[1];
+    int S = pick_splits((Mrows + 63) / 64, d[2] * d[3] / 64, 64);
+    if (S == 1) gemm_cm_f64(s, w.x64(), w.x32(), Mrows, d[2] * d[3], k, w.ldr, 1, w.YL);
+    else {
+      REQUIRE(S * Mrows * w.ldr <= w.part_cap, PTN_ERR_INVALID, "partial buffer too small");
+      gemm_cm_f64(s, w.x64(), w.x32(), Mrows, d[2] * d[3], k, w.ldr, S, w.part);
+      reduce_splits(s, w.part, S, Mrows * w.ldr, w.YL);
+    }
+  }
+  if (mode == 2) {
+    KrSpec k{};
+    k.nm = 2; k.dim[0] = d[0]; k.dim[1] = d[1]; k.fac[0] = w.Fb(base, 0); k.fac[1] = w.Fb(base, 1);
+    gemm_ck_f64(s, w.x64(), w.x32(), d[0] * d[1], d[2] * d[3], k, w.ldr, w.T);
+  }
+  const bool left = mode <= 1;
+  const double *node = left ? w.YL.p : w.T.p;
+  KrSpec sp{};
+  sp.nm = 2;
+  int m0 = left ? 0 : 2;
+  sp.dim[0] = d[m0]; sp.dim[1] = d[m0 + 1]; sp.fac[0] = w.Fb(base, m0); sp.fac[1] = w.Fb(base, m0 + 1);
+  int om = mode - m0;
+  int S = pick_splits(d[mode], d[m0 + 1 - om] / 64, 64);
+  contract_rows(s, node, nullptr, w.ldr, sp, om, w.ldr, S, w.part);
+  reduce_splits(s, w.part, S, d[mode] * w.ldr, M);
+}
+
+// Hadamard order of V_n: ascending (PartialCwiseProd, right-nested) for the standard
+// form; the tree path order (TreeMode_N_Product / TTVs_util) for the dimension tree.
+int v_order(int N, int mode, bool tree, int *list) {
+  if (tree && N == 3) {
+    static const int o3[3][2] = {{2, 1}, {2, 0}, {0, 1}};
+    list[0] = o3[mode][0]; list[1] = o3[mode][1];
+    return 2;
+  }
+  if (tree && N == 4) {
+    static const int o4[4][3] = {{3, 2, 1}, {3, 2, 0}, {0, 1, 3}, {0, 1, 2}};
+    for (int j = 0; j < 3; ++j) list[j] = o4[mode][j];
+    return 3;
+  }
+  int n = 0;
+  for (int k = 0; k < N; ++k)
+    if (k != mode) list[n++] = k;
+  return n;
+}
+
+void compute_V(Work &w, double *gbase, int mode, bool tree) {
+  int list[kMaxOrder];
+  int n = v_order(w.N, mode, tree, list);
+  const double *gp[kMaxOrder];
+  for (int j = 0; j < n; ++j) gp[j] = gbase + (int64_t)list[j] * w.R * w.R;
+  hadamard(w.h->stream, w.R, n, gp, w.V);
+}
+
+void gram_of(Work &w, const double *A, int64_t I, double *G) {
+  int np = (int)std::min<int64_t>(w.gparts, (I + 7) / 8);
+  if (np < 1) np = 1;
+  gram_partials(w.h->stream, I, w.R, w.ldr, A, w.gpart, np);
+  gram_reduce(w.h->stream, w.R, w.gpart, np, G);
+}
+
+double read_scalar(ptn_handle h, const double *dev) {
+  double v;
+  HIPCHK(hipMemcpyAsync(&v, dev, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return v;
+}
+
+void frob_norm(ptn_handle h) {
+  if (h->frob_valid) return;
+  square_norm_partials(h->stream, h->total, h->x64, h->x64 ? nullptr : h->x32, h->part, kNormParts);
+  sum_partials(h->stream, h->part, kNormParts, h->scal);
+  h->frob_host = read_scalar(h, h->scal);
+  h->frob_valid = true;
+}
+
+void upload_factors(Work &w, const double *packed_host, double *dst_rows) {
+  ptn_handle h = w.h;
+  int64_t n = 0;
+  for (int k = 0; k < w.N; ++k) n += w.dims[k] * w.R;
+  DevBuf<double> tmp(h, n);
+  HIPCHK(hipMemcpyAsync(tmp.p, packed_host, n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  int64_t o = 0;
+  for (int k = 0; k < w.N; ++k) {
+    colmajor_to_rows(h->stream, w.dims[k], w.R, w.ldr, tmp.p + o, dst_rows + w.off[k]);
+    o += w.dims[k] * w.R;
+  }
+  HIPCHK(hipStreamSynchronize(h->stream));
+}
+
+void download_factors(Work &w, const double *src_rows, double *packed_host) {
+  ptn_handle h = w.h;
+  int64_t n = 0;
+  for (int k = 0; k < w.N; ++k) n += w.dims[k] * w.R;
+  DevBuf<double> tmp(h, n);
+  int64_t o = 0;
+  for (int k = 0; k < w.N; ++k) {
+    rows_to_colmajor(h->stream, w.dims[k], w.R, w.ldr, src_rows + w.off[k], tmp.p + o);
+    o += w.dims[k] * w.R;
+  }
+  HIPCHK(hipMemcpyAsync(packed_host, tmp.p, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+}
+
+double cost_explicit(Work &w, const double *base) {
+  ptn_handle h = w.h;
+  const double *fp[kMaxOrder];
+  for (int k = 0; k < w.N; ++k) fp[k] = base + w.off[k];
+  cost_explicit_partials(h->stream, w.N, w.dims, w.R, w.ldr, fp, h->x64, h->x64 ? nullptr : h->x32, w.part,
+                         kNormParts);
+  sum_partials(h->stream, w.part, kNormParts, w.scal.p + 3);
+  return std::sqrt(read_scalar(h, w.scal.p + 3));
+}
+
+// update_factor, Cpd.hpp:189-228, nonnegativity branch on the device
+void update_nonneg(Work &w, int mode, const ptn_options &o) {
+  hipStream_t s = w.h->stream;
+  const int64_t I = w.dims[mode];
+  double *A = w.F(mode);
+  NnlsParams *p = w.prm.p + mode;
+  ptn_handle h = w.h;
+  HIPCHK(hipMemcpyAsync(w.Aold, A, I * w.ldr * sizeof(double), hipMemcpyDeviceToDevice, s));
+  eig_extremes(s, w.R, w.V, p);
+  nnls_init_mask(s, w.mask, w.colsum, w.R);
+  nnls_pass1(s, I, w.R, w.ldr, w.V, p, w.M, w.Aold, o.nesterov_delta_1, o.nesterov_delta_2, o.max_inner, w.mask);
+  nnls_pass2(s, I, w.R, w.ldr, w.V, p, w.M, w.Aold, A, o.max_inner, w.mask, w.colsum);
+  zero_col_fallback(s, I, w.R, w.ldr, w.colsum, w.Aold, A, p);
+}
+
+}  // namespace
+
+// ======================================================================================
+extern "C" {
+
+int ptn_abi_version(void) { return 1; }
+
+void ptn_default_options(ptn_options *o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->max_iter = 20;
+  o->threshold_error = 1e-3;
+  o->nesterov_delta_1 = 1e-2;
+  o->nesterov_delta_2 = 1e-2;
+  o->accel_coeff = 3;
+  o->accel_fail = 0;
+  o->acceleration = 1;
+  o->normalization = 1;
+  for (int i = 0; i < PTN_MAX_ORDER; ++i) o->constraints[i] = PTN_NONNEGATIVITY;
+  o->mttkrp_form = PTN_MTTKRP_STANDARD;
+  o->arith = PTN_ARITH_F64;
+  o->max_inner = 50;
+  o->fuse_accel = 0;
+}
+
+ptn_status ptn_create(ptn_handle *out, int device) {
+  if (!out) return PTN_ERR_INVALID;
+  *out = nullptr;
+  ptn_handle h = new ptn_handle_s();
+  h->device = device;
+  ptn_status st = run(h, [&] {
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    REQUIRE(device >= 0 && device < n, PTN_ERR_INVALID, "no such device");
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    HIPCHK(hipMalloc((void **)&h->scal, 16 * sizeof(double)));
+    HIPCHK(hipMalloc((void **)&h->part, kNormParts * sizeof(double)));
+  });
+  if (st != PTN_OK) {
+    ptn_destroy(h);
+    return st;
+  }
+  *out = h;
+  return PTN_OK;
+}
+
+void ptn_destroy(ptn_handle h) {
+  if (!h) return;
+  (void)hipSetDevice(h->device);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->x64) (void)hipFree(h->x64);
+  if (h->x32) (void)hipFree(h->x32);
+  if (h->scal) (void)hipFree(h->scal);
+  if (h->part) (void)hipFree(h->part);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+}
+
+const char *ptn_last_error(ptn_handle h) { return h ? h->err.c_str() : "null handle"; }
+
+}  // extern "C"
+
+namespace {
+
+ptn_status run(ptn_handle h, const std::function<void()> &fn) {
+  if (!h) return PTN_ERR_INVALID;
+  try {
+    if (h->stream) {
+      hipError_t e = hipSetDevice(h->device);
+      if (e != hipSuccess) {
+        h->err = hipGetErrorString(e);
+        return PTN_ERR_HIP;
+      }
+    }
+    fn();
+    return PTN_OK;
+  } catch (const Fail &f) {
+    return f.code;
+  } catch (const std::bad_alloc &) {
+    h->err = "host allocation failed";
+    return PTN_ERR_OOM;
+  } catch (...) {
+    h->err = "unexpected exception";
+    return PTN_ERR_HIP;
+  }
+}
+
+void alloc_tensor(ptn_handle h, int N, const int64_t *dims, int storage) {
+  REQUIRE(storage >= 1 && storage <= 3, PTN_ERR_INVALID, "storage must be F64, F32 or BOTH");
+  check_dims(h, N, dims);
+  if (h->x64) { HIPCHK(hipFree(h->x64)); h->x64 = nullptr; }
+  if (h->x32) { HIPCHK(hipFree(h->x32)); h->x32 = nullptr; }
+  h->N = N;
+  for (int k = 0; k < N; ++k) h->dims[k] = dims[k];
+  h->total = prod(dims, N);
+  h->frob_valid = false;
+  if (storage & PTN_TENSOR_F64) HIPCHK(hipMalloc((void **)&h->x64, h->total * sizeof(double)));
+  if (storage & PTN_TENSOR_F32) HIPCHK(hipMalloc((void **)&h->x32, h->total * sizeof(float)));
+}
+
+void check_arith(ptn_handle h, int arith) {
+  REQUIRE(arith >= PTN_ARITH_F64 && arith <= 3, PTN_ERR_INVALID, "bad arith");
+  REQUIRE(h->N > 0, PTN_ERR_STATE, "no tensor set on the handle");
+  if (arith == PTN_ARITH_F64) REQUIRE(h->x64, PTN_ERR_STATE, "FP64 arithmetic needs the FP64 tensor copy");
+  else REQUIRE(h->x32, PTN_ERR_STATE, "FP32 tensor streams need the FP32 tensor copy");
+  REQUIRE(arith == PTN_ARITH_F64 || arith == 3, PTN_ERR_UNSUPPORTED, "arith mode not built yet");
+}
+
+}  // namespace
+
+extern "C" {
+
+ptn_status ptn_set_tensor(ptn_handle h, int N, const int64_t *dims, const double *X, int storage) {
+  return run(h, [&] {
+    REQUIRE(X != nullptr, PTN_ERR_INVALID, "X is null");
+    alloc_tensor(h, N, dims, storage);
+    if (h->x64) {
+      HIPCHK(hipMemcpyAsync(h->x64, X, h->total * sizeof(double), hipMemcpyHostToDevice, h->stream));
+      if (h->x32) cast_f64_to_f32(h->stream, h->total, h->x64, h->x32);
+    } else {
+      DevBuf<double> tmp(h, h->total);
+      HIPCHK(hipMemcpyAsync(tmp.p, X, h->total * sizeof(double), hipMemcpyHostToDevice, h->stream));
+      cast_f64_to_f32(h->stream, h->total, tmp.p, h->x32);
+      HIPCHK(hipStreamSynchronize(h->stream));
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+  });
+}
+
+ptn_status ptn_generate_tensor(ptn_handle h, int N, const int64_t *dims, int R_true, uint64_t seed, int dist,
+                               double snr_db, int storage) {
+  return run(h, [&] {
+    REQUIRE(R_true >= 1 && R_true <= 4096, PTN_ERR_INVALID, "bad true rank");
+    REQUIRE(dist == 0 || dist == 1, PTN_ERR_INVALID, "dist must be 0 (uniform) or 1 (exponential)");
+    alloc_tensor(h, N, dims, storage);
+    int64_t nf = 0;
+    for (int k = 0; k < N; ++k) nf += dims[k] * R_true;
+    DevBuf<double> fac(h, nf);
+    gen_factors(h->stream, N, dims, R_true, seed, dist, fac.p);
+    cp_full(h->stream, N, dims, R_true, fac.p, h->x64, h->x32);
+    if (!std::isnan(snr_db)) {
+      square_norm_partials(h->stream, h->total, h->x64, h->x64 ? nullptr : h->x32, h->part, kNormParts);
+      sum_partials(h->stream, h->part, kNormParts, h->scal + 1);
+      gen_noise_norm(h->stream, h->total, seed, h->part, kNormParts);
+      sum_partials(h->stream, h->part, kNormParts, h->scal + 2);
+      noise_scale(h->stream, h->scal + 1, h->scal + 2, snr_db, h->scal + 3);
+      add_noise(h->stream, h->total, seed, h->scal + 3, h->x64, h->x32);
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipGetLastError());
+  });
+}
+
+ptn_status ptn_generate_factors(ptn_handle h, int N, const int64_t *dims, int R, uint64_t seed, int dist,
+                                double *packed_out) {
+  return run(h, [&] {
+    check_dims(h, N, dims);
+    REQUIRE(packed_out, PTN_ERR_INVALID, "output is null");
+    int64_t nf = 0;
+    for (int k = 0; k < N; ++k) nf += dims[k] * R;
+    DevBuf<double> fac(h, nf);
+    gen_factors(h->stream, N, dims, R, seed, dist, fac.p);
+    HIPCHK(hipMemcpyAsync(packed_out, fac.p, nf * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+  });
+}
+
+ptn_status ptn_tensor_norm_sq(ptn_handle h, double *out) {
+  return run(h, [&] {
+    REQUIRE(h->N > 0, PTN_ERR_STATE, "no tensor set on the handle");
+    frob_norm(h);
+    *out = h->frob_host;
+  });
+}
+
+ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_options *opt, double *out_packed,
+                   ptn_result *res, ptn_sweep_log *log, int log_cap) {
+  return run(h, [&] {
+    REQUIRE(opt && init_packed && out_packed && res, PTN_ERR_INVALID, "null argument");
+    const ptn_options &o = *opt;
+    check_arith(h, o.arith);
+    REQUIRE(R >= 1 && R <= PTN_MAX_RANK, PTN_ERR_UNSUPPORTED, "rank must be in [1, 64]");
+    REQUIRE(o.max_iter >= 1, PTN_ERR_INVALID, "max_iter must be >= 1");
+    REQUIRE(o.max_inner >= 0 && o.max_inner <= kMaxInner, PTN_ERR_UNSUPPORTED, "max_inner must be in [0, 255]");
+    REQUIRE(o.accel_coeff >= 1, PTN_ERR_INVALID, "accel_coeff must be >= 1");
+    const bool tree = o.mttkrp_form == PTN_MTTKRP_DIMTREE;
+    REQUIRE(o.mttkrp_form == PTN_MTTKRP_STANDARD || tree, PTN_ERR_INVALID, "bad mttkrp_form");
+    REQUIRE(!tree || h->N == 3 || h->N == 4, PTN_ERR_UNSUPPORTED, "dimension tree built for N = 3, 4");
+    for (int k = 0; k < h->N; ++k)
+      REQUIRE(o.constraints[k] == PTN_NONNEGATIVITY, PTN_ERR_UNSUPPORTED, "only the nonnegativity constraint is built");
+
+    Work w(h, R, o.arith, tree);
+    const int N = w.N, last = N - 1;
+    upload_factors(w, init_packed, w.fac);
+    for (int k = 0; k < N; ++k) gram_of(w, w.F(k), w.dims[k], w.G(k));
+
+    // choose_normilization_factor, Normalize.hpp:53-67
+    int wf = 0;
+    bool all_orth = true;
+    if (o.normalization)
+      for (int k = 0; k < N; ++k)
+        if (o.constraints[k] != PTN_ORTHOGONALITY) { all_orth = false; wf = k; break; }
+
+    frob_norm(h);
+    ptn_result r{};
+    r.frob_sq = h->frob_host;
+    r.f_value = cost_explicit(w, w.fac);   // cost_function_init
+    r.rel_cost = r.f_value / std::sqrt(r.frob_sq);
+    r.accel_coeff = o.accel_coeff;
+    r.accel_fail = o.accel_fail;
+    HIPCHK(hipMemcpyAsync(w.scal.p, &r.frob_sq, sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+
+    auto t0 = std::chrono::steady_clock::now();
+    NnlsParams hp[kMaxOrder];
+    for (;;) {
+      r.ao_iter++;
+      ptn_sweep_log lg{};
+      for (int n = 0; n < N; ++n) {
+        compute_V(w, w.grams, n, tree);
+        if (tree) mttkrp_tree(w, w.fac, n, w.M);
+        else mttkrp_standard(w, w.fac, n, w.M);
+        update_nonneg(w, n, o);
+        gram_of(w, w.F(n), w.dims[n], w.G(n));
+        if (n == last) {
+          dot_partials(h->stream, w.dims[last] * w.ldr, w.M, w.F(last), w.dpart, w.dparts);
+          cost_gram_final(h->stream, R, w.scal.p, w.dpart, w.dparts, w.V, w.G(last), w.scal.p + 1);
+        }
+      }
+      if (o.normalization && !all_orth) {
+        double *gp[kMaxOrder], *fp[kMaxOrder];
+        for (int k = 0; k < N; ++k) { gp[k] = w.G(k); fp[k] = w.F(k); }
+        normalize(h->stream, N, w.dims, R, w.ldr, wf, gp, fp, w.nscr);
+      }
+      HIPCHK(hipMemcpyAsync(hp, w.prm.p, sizeof(NnlsParams) * N, hipMemcpyDeviceToHost, h->stream));
+      r.f_value = read_scalar(h, w.scal.p + 1);
+      r.rel_cost = r.f_value / std::sqrt(r.frob_sq);
+      lg.f_value = r.f_value;
+      lg.rel_cost = r.rel_cost;
+      for (int n = 0; n < N; ++n) {
+        lg.nnls_iters[n] = hp[n].iters;
+        lg.zero_col_fallback[n] = hp[n].fallback;
+        lg.lipschitz[n] = hp[n].L0;
+        lg.mu[n] = hp[n].mu0;
+      }
+      const bool stop = r.rel_cost < o.threshold_error || r.ao_iter >= o.max_iter;
+      if (!stop && o.acceleration) {
+        HIPCHK(hipMemcpyAsync(w.nrm, w.fac, w.fsz * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+        if (r.ao_iter > 1) {
+          // line_search_accel, Cpd.hpp:247-278
+          double step = std::pow((double)(r.ao_iter + 1), 1.0 / (double)r.accel_coeff);
+          axpy_accel(h->stream, w.fsz, step, w.old, w.fac, w.acc);
+          for (int k = 0; k < N; ++k) gram_of(w, w.Fb(w.acc, k), w.dims[k], w.agr.p + (int64_t)k * R * R);
+          mttkrp_standard(w, w.acc, last, w.M);
+          int list[kMaxOrder];
+          int nl = v_order(N, last, false, list);
+          const double *gp[kMaxOrder];
+          for (int j = 0; j < nl; ++j) gp[j] = w.agr.p + (int64_t)list[j] * R * R;
+          hadamard(h->stream, R, nl, gp, w.V);
+          dot_partials(h->stream, w.dims[last] * w.ldr, w.M, w.Fb(w.acc, last), w.dpart, w.dparts);
+          accel_cost_final(h->stream, R, w.scal.p, w.dpart, w.dparts, w.V, w.agr.p + (int64_t)last * R * R,
+                           w.scal.p + 2);
+          double fa = read_scalar(h, w.scal.p + 2);
+          lg.accel_tried = 1;
+          lg.f_accel = fa;
+          if (r.f_value > fa) {
+            HIPCHK(hipMemcpyAsync(w.fac, w.acc, w.fsz * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+            HIPCHK(hipMemcpyAsync(w.grams, w.agr, (size_t)N * R * R * sizeof(double), hipMemcpyDeviceToDevice,
+                                  h->stream));
+            r.f_value = fa;
+            lg.accel_accepted = 1;
+          } else {
+            r.accel_fail++;
+          }
+          if (r.accel_fail == 5) {
+            r.accel_fail = 0;
+            r.accel_coeff++;
+          }
+        }
+        HIPCHK(hipMemcpyAsync(w.old, w.nrm, w.fsz * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+      }
+      if (log && r.ao_iter - 1 < log_cap) log[r.ao_iter - 1] = lg;
+      if (stop) break;
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    r.sweep_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    HIPCHK(hipGetLastError());
+    download_factors(w, w.fac, out_packed);
+    *res = r;
+  });
+}
+
+ptn_status ptn_mttkrp(ptn_handle h, int R, const double *factors_packed, int mode, int arith, double *M_out) {
+  return run(h, [&] {
+    check_arith(h, arith);
+    REQUIRE(R >= 1 && R <= PTN_MAX_RANK, PTN_ERR_UNSUPPORTED, "rank must be in [1, 64]");
+    REQUIRE(mode >= 0 && mode < h->N, PTN_ERR_INVALID, "bad mode");
+    REQUIRE(factors_packed && M_out, PTN_ERR_INVALID, "null argument");
+    Work w(h, R, arith, false);
+    upload_factors(w, factors_packed, w.fac);
+    mttkrp_standard(w, w.fac, mode, w.M);
+    DevBuf<double> cm(h, w.dims[mode] * R);
+    rows_to_colmajor(h->stream, w.dims[mode], R, w.ldr, w.M, cm.p);
+    HIPCHK(hipMemcpyAsync(M_out, cm.p, w.dims[mode] * R * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipGetLastError());
+  });
+}
+
+ptn_status ptn_mttkrp_dimtree(ptn_handle h, int R, const double *factors_packed, int arith, double *M_packed_out) {
+  return run(h, [&] {
+    check_arith(h, arith);
+    REQUIRE(R >= 1 && R <= PTN_MAX_RANK, PTN_ERR_UNSUPPORTED, "rank must be in [1, 64]");
+    REQUIRE(h->N == 3 || h->N == 4, PTN_ERR_UNSUPPORTED, "dimension tree built for N = 3, 4");
+    REQUIRE(factors_packed && M_packed_out, PTN_ERR_INVALID, "null argument");
+    Work w(h, R, arith, true);
+    upload_factors(w, factors_packed, w.fac);
+    int64_t o = 0;
+    for (int n = 0; n < w.N; ++n) {
+      mttkrp_tree(w, w.fac, n, w.M);
+      DevBuf<double> cm(h, w.dims[n] * R);
+      rows_to_colmajor(h->stream, w.dims[n], R, w.ldr, w.M, cm.p);
+      HIPCHK(hipMemcpyAsync(M_packed_out + o, cm.p, w.dims[n] * R * sizeof(double), hipMemcpyDeviceToHost,
+                            h->stream));
+      HIPCHK(hipStreamSynchronize(h->stream));
+      o += w.dims[n] * R;
+    }
+    HIPCHK(hipGetLastError());
+  });
+}
+
+ptn_status ptn_sym_extremes(ptn_handle h, int R, const double *V, double *L, double *mu) {
+  return run(h, [&] {
+    REQUIRE(R >= 1 && R <= PTN_MAX_RANK, PTN_ERR_UNSUPPORTED, "rank must be in [1, 64]");
+    REQUIRE(V && L && mu, PTN_ERR_INVALID, "null argument");
+    DevBuf<double> dV(h, (size_t)R * R);
+    DevBuf<NnlsParams> p(h, 1);
+    HIPCHK(hipMemcpyAsync(dV.p, V, (size_t)R * R * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    eig_extremes(h->stream, R, dV.p, p.p);
+    NnlsParams hp;
+    HIPCHK(hipMemcpyAsync(&hp, p.p, sizeof(hp), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    *L = hp.L0;
+    *mu = hp.mu0;
+  });
+}
+
+ptn_status ptn_nesterov_mnls(ptn_handle h, int64_t I, int R, const double *Z, const double *W, double delta_1,
+                             double delta_2, int max_inner, double *A, int *iters, double *L, double *mu) {
+  return run(h, [&] {
+    REQUIRE(R >= 1 && R <= PTN_MAX_RANK, PTN_ERR_UNSUPPORTED, "rank must be in [1, 64]");
+    REQUIRE(I >= 1 && I < (1ll << 31), PTN_ERR_INVALID, "bad row count");
+    REQUIRE(max_inner >= 0 && max_inner <= kMaxInner, PTN_ERR_UNSUPPORTED, "max_inner must be in [0, 255]");
+    REQUIRE(Z && W && A, PTN_ERR_INVALID, "null argument");
+    const int ldr = round_up16(R);
+    DevBuf<double> dZ(h, (size_t)R * R), tmp(h, I * R), dW(h, I * ldr), dA(h, I * ldr), dB(h, I * ldr);
+    DevBuf<double> colsum(h, kMaxRank);
+    DevBuf<NnlsParams> p(h, 1);
+    DevBuf<uint64_t> mask(h, 4);
+    std::vector<double> zr((size_t)R * R);
+    for (int a = 0; a < R; ++a)
+      for (int b = 0; b < R; ++b) zr[a * R + b] = Z[a + (int64_t)R * b];
+    HIPCHK(hipMemcpyAsync(dZ.p, zr.data(), zr.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(tmp.p, W, I * R * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    colmajor_to_rows(h->stream, I, R, ldr, tmp.p, dW.p);
+    HIPCHK(hipMemcpyAsync(tmp.p, A, I * R * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    colmajor_to_rows(h->stream, I, R, ldr, tmp.p, dA.p);
+    eig_extremes(h->stream, R, dZ.p, p.p);
+    nnls_init_mask(h->stream, mask.p, colsum.p, R);
+    nnls_pass1(h->stream, I, R, ldr, dZ.p, p.p, dW.p, dA.p, delta_1, delta_2, max_inner, mask.p);
+    nnls_pass2(h->stream, I, R, ldr, dZ.p, p.p, dW.p, dA.p, dB.p, max_inner, mask.p, colsum.p);
+    rows_to_colmajor(h->stream, I, R, ldr, dB.p, tmp.p);
+    HIPCHK(hipMemcpyAsync(A, tmp.p, I * R * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    NnlsParams hp;
+    HIPCHK(hipMemcpyAsync(&hp, p.p, sizeof(hp), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipGetLastError());
+    if (iters) *iters = hp.iters;
+    if (L) *L = hp.L0;
+    if (mu) *mu = hp.mu0;
+  });
+}
+
+ptn_status ptn_normalize(ptn_handle h, int N, const int64_t *dims, int R, int weight_factor, double *grams_packed,
+                         double *factors_packed) {
+  return run(h, [&] {
+    check_dims(h, N, dims);
+    REQUIRE(R >= 1 && R <= PTN_MAX_RANK, PTN_ERR_UNSUPPORTED, "rank must be in [1, 64]");
+    REQUIRE(weight_factor >= 0 && weight_factor < N, PTN_ERR_INVALID, "bad weight factor");
+    REQUIRE(grams_packed && factors_packed, PTN_ERR_INVALID, "null argument");
+    const int ldr = round_up16(R);
+    int64_t off[kMaxOrder], o = 0, nf = 0;
+    for (int k = 0; k < N; ++k) { off[k] = o; o += dims[k] * ldr; nf += dims[k] * R; }
+    DevBuf<double> fac(h, o), tmp(h, nf), G(h, (size_t)N * R * R), scr(h, 2 * kMaxOrder * kMaxRank);
+    std::vector<double> gr((size_t)N * R * R);
+    for (int k = 0; k < N; ++k)
+      for (int a = 0; a < R; ++a)
+        for (int b = 0; b < R; ++b) gr[(size_t)k * R * R + a * R + b] = grams_packed[(size_t)k * R * R + a + R * b];
+    HIPCHK(hipMemcpyAsync(G.p, gr.data(), gr.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(tmp.p, factors_packed, nf * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    int64_t oc = 0;
+    double *gp[kMaxOrder], *fp[kMaxOrder];
+    for (int k = 0; k < N; ++k) {
+      colmajor_to_rows(h->stream, dims[k], R, ldr, tmp.p + oc, fac.p + off[k]);
+      oc += dims[k] * R;
+      gp[k] = G.p + (int64_t)k * R * R;
+      fp[k] = fac.p + off[k];
+    }
+    normalize(h->stream, N, dims, R, ldr, weight_factor, gp, fp, scr.p);
+    oc = 0;
+    for (int k = 0; k < N; ++k) {
+      rows_to_colmajor(h->stream, dims[k], R, ldr, fac.p + off[k], tmp.p + oc);
+      oc += dims[k] * R;
+    }
+    HIPCHK(hipMemcpyAsync(factors_packed, tmp.p, nf * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(gr.data(), G.p, gr.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipGetLastError());
+    for (int k = 0; k < N; ++k)
+      for (int a = 0; a < R; ++a)
+        for (int b = 0; b < R; ++b) grams_packed[(size_t)k * R * R + a + R * b] = gr[(size_t)k * R * R + a * R + b];
+  });
+}
+
+ptn_status ptn_cost_explicit(ptn_handle h, int R, const double *factors_packed, double *f) {
+  return run(h, [&] {
+    REQUIRE(h->N > 0, PTN_ERR_STATE, "no tensor set on the handle");
+    REQUIRE(R >= 1 && R <= PTN_MAX_RANK, PTN_ERR_UNSUPPORTED, "rank must be in [1, 64]");
+    REQUIRE(factors_packed && f, PTN_ERR_INVALID, "null argument");
+    Work w(h, R, h->x64 ? PTN_ARITH_F64 : 3, false);
+    upload_factors(w, factors_packed, w.fac);
+    *f = cost_explicit(w, w.fac);
+  });
+}
+
+ptn_status ptn_time_contraction(ptn_handle h, int kernel, int R, const double *factors_packed, int arith, int iters,
+                                double *avg_ms, double *bytes_out) {
+  return run(h, [&] {
+    check_arith(h, arith);
+    REQUIRE(R >= 1 && R <= PTN_MAX_RANK, PTN_ERR_UNSUPPORTED, "rank must be in [1, 64]");
+    REQUIRE(kernel >= 0 && kernel <= 2 && iters >= 1 && avg_ms, PTN_ERR_INVALID, "bad argument");
+    REQUIRE(h->N == 3 || kernel == 2, PTN_ERR_UNSUPPORTED, "node timing built for N = 3");
+    Work w(h, R, arith, true);
+    upload_factors(w, factors_packed, w.fac);
+    const int64_t *d = w.dims;
+    const double sx = arith == PTN_ARITH_F64 ? 8.0 : 4.0;
+    auto launch = [&] {
+      if (kernel == 0) {
+        KrSpec k{};
+        k.nm = 1; k.dim[0] = d[2]; k.fac[0] = w.F(2);
+        gemm_cm_f64(h->stream, w.x64(), w.x32(), d[0] * d[1], d[2], k, w.ldr, 1, w.YL);
+      } else if (kernel == 1) {
+        KrSpec k0{};
+        k0.nm = 1; k0.dim[0] = d[0]; k0.fac[0] = w.F(0);
+        gemm_ck_f64(h->stream, w.x64(), w.x32(), d[0], w.total / d[0], k0, w.ldr, w.T);
+      } else {
+        mttkrp_standard(w, w.fac, 0, w.M);
+      }
+    };
+    launch();
+    HIPCHK(hipStreamSynchronize(h->stream));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, h->stream));
+    for (int i = 0; i < iters; ++i) launch();
+    HIPCHK(hipEventRecord(e1, h->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIPCHK(hipGetLastError());
+    *avg_ms = ms / iters;
+    if (bytes_out) {
+      // algorithmic bytes: tensor once + output + factor reads
+      double b = sx * (double)w.total;
+      if (kernel == 0) b += 8.0 * R * (double)(d[0] * d[1]) + 8.0 * R * d[2];
+      else if (kernel == 1) b += 8.0 * R * (double)(w.total / d[0]) + 8.0 * R * d[0];
+      else b += 8.0 * R * d[0] + 8.0 * R * (double)(d[1] + d[2]);
+      *bytes_out = b;
+    }
+  });
+}
+
+}  // extern "C"

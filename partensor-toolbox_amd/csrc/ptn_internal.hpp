@@ -1,0 +1,100 @@
+// Internal declarations shared by the HIP kernels and the C-ABI layer.
+//
+// Device layouts (all chosen for the GPU, converted at the ABI boundary):
+//   tensor X      column-major, mode 0 fastest (as the reference's Eigen Tensor)
+//   factor A_k    row-major I_k x ldr doubles (ldr = R rounded up to 16, padding 0),
+//                 so one factor row is contiguous for the KR products and NNLS rows
+//   MTTKRP M_n    row-major I_n x ldr doubles
+//   Gram / V      row-major R x R doubles (symmetric)
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace ptn {
+
+constexpr int kMaxOrder = 8;
+constexpr int kMaxRank = 64;        // one NNLS row per 64-lane wavefront
+constexpr int kMaxInner = 255;      // satisfaction masks hold 4 x 64 bits
+
+// Mixed-radix description of an index set enumerated lowest-mode-fastest,
+// with the factor row products attached to it (a Khatri-Rao row generator).
+struct KrSpec {
+  int nm;                          // number of modes in the set
+  int64_t dim[kMaxOrder];          // their sizes
+  int64_t xstride[kMaxOrder];      // their strides in the tensor X (elements)
+  const double *fac[kMaxOrder];    // row-major factor of each mode (nullptr: not multiplied)
+  const float *fac32[kMaxOrder];   // FP32 copies (bf16x3 / FP32 kernels)
+};
+
+// Per-mode NNLS constants produced on the device by the eigen kernel.
+struct NnlsParams {
+  double L0, mu0;       // extreme singular values of V (before GLambda)
+  double L, lambda, q;  // after GLambda
+  int iters;            // projected-gradient steps taken (k*)
+  int fallback;         // zero-column blend applied
+};
+
+// ---- generation (gen_kernels.hip) ----
+void gen_factors(hipStream_t s, int N, const int64_t *dims, int R, uint64_t seed, int dist, double *packed);
+void cp_full(hipStream_t s, int N, const int64_t *dims, int R, const double *packed_dev,
+             double *x64, float *x32);
+void gen_noise_norm(hipStream_t s, int64_t n, uint64_t seed, double *partials, int nparts);
+void add_noise(hipStream_t s, int64_t n, uint64_t seed, const double *scale_dev, double *x64, float *x32);
+void square_norm_partials(hipStream_t s, int64_t n, const double *x64, const float *x32, double *partials, int nparts);
+void sum_partials(hipStream_t s, const double *partials, int nparts, double *out);
+void noise_scale(hipStream_t s, const double *nx2, const double *ne2, double snr_db, double *scale);
+void cast_f64_to_f32(hipStream_t s, int64_t n, const double *src, float *dst);
+void cost_explicit_partials(hipStream_t s, int N, const int64_t *dims, int R, int ldr, const double *const *fac,
+                            const double *x64, const float *x32, double *partials, int nparts);
+
+// ---- layout conversion ----
+void colmajor_to_rows(hipStream_t s, int64_t I, int R, int ldr, const double *src, double *dst);
+void rows_to_colmajor(hipStream_t s, int64_t I, int R, int ldr, const double *src, double *dst);
+void rows_to_f32(hipStream_t s, int64_t n, const double *src, float *dst);
+
+// ---- MTTKRP building blocks (mttkrp_kernels.hip) ----
+// out[split][m][r] = sum_{k in split} X[m + Mrows*k] * W(k)[r];  W = Khatri-Rao row of `w`.
+void gemm_cm_f64(hipStream_t s, const double *x64, const float *x32, int64_t Mrows, int64_t Ktot,
+                 const KrSpec &w, int ldr, int splits, double *out);
+// T[c][r] = sum_{a < L} X[a + L*c] * W(a)[r]   for c < C;  W = Khatri-Rao row of `w` (a < 2^32)
+void gemm_ck_f64(hipStream_t s, const double *x64, const float *x32, int64_t L, int64_t C,
+                 const KrSpec &w, int ldr, double *T);
+// out[i][r] (+)= sum over rows t of T whose output digit is i of T[t][r] * prod of the
+// other digits' factor rows.  `t` enumerates spec modes lowest fastest; `outmode` is the
+// position in spec of the output index.  Rows of T are ldt apart; T may be FP64 or FP32.
+void contract_rows(hipStream_t s, const double *T64, const float *T32, int64_t ldt, const KrSpec &spec,
+                   int outmode, int ldr, int splits, double *partials);
+void reduce_splits(hipStream_t s, const double *partials, int splits, int64_t n, double *out);
+
+// FP32-stream MFMA variants (bf16x3 split products or FP32 MFMA), FP32 outputs.
+enum GemmMode { kGemmFp32 = 0, kGemmBf16x3 = 1 };
+void gemm_cm_f32(hipStream_t s, GemmMode mode, const float *x32, int64_t Mrows, int64_t Ktot,
+                 const KrSpec &w, int ldr, int splits, float *out);
+void gemm_ck_f32(hipStream_t s, GemmMode mode, const float *x32, int64_t L, int64_t C,
+                 const float *A0, int ldr, float *T);
+
+// ---- small dense pieces (small_kernels.hip) ----
+void hadamard(hipStream_t s, int R, int ng, const double *const *grams, double *V);
+void gram_partials(hipStream_t s, int64_t I, int R, int ldr, const double *A, double *partials, int nparts);
+void gram_reduce(hipStream_t s, int R, const double *partials, int nparts, double *G);
+void eig_extremes(hipStream_t s, int R, const double *V, NnlsParams *p);
+void nnls_init_mask(hipStream_t s, uint64_t *mask, double *colsum, int R);
+void nnls_pass1(hipStream_t s, int64_t I, int R, int ldr, const double *V, const NnlsParams *p,
+                const double *M, const double *A, double d1, double d2, int max_inner, uint64_t *mask);
+void nnls_pass2(hipStream_t s, int64_t I, int R, int ldr, const double *V, NnlsParams *p,
+                const double *M, const double *Aold, double *Anew, int max_inner, const uint64_t *mask,
+                double *colsum);
+void zero_col_fallback(hipStream_t s, int64_t I, int R, int ldr, const double *colsum, const double *Aold,
+                       double *A, NnlsParams *p);
+void unconstrained_update(hipStream_t s, int64_t I, int R, int ldr, const double *V, const double *M, double *A,
+                          double *scratch);
+void dot_partials(hipStream_t s, int64_t n, const double *a, const double *b, double *partials, int nparts);
+void cost_gram_final(hipStream_t s, int R, const double *frob, const double *ma_partials, int nparts,
+                     const double *V, const double *G, double *f);
+void accel_cost_final(hipStream_t s, int R, const double *frob, const double *ma_partials, int nparts,
+                      const double *V, const double *G, double *f);
+void normalize(hipStream_t s, int N, const int64_t *dims, int R, int ldr, int wf, double *const *grams,
+               double *const *fac, double *scratch);
+void axpy_accel(hipStream_t s, int64_t n, double step, const double *old, const double *cur, double *out);
+
+}  // namespace ptn

@@ -1,0 +1,284 @@
+"""Python binding of libptn_hip.so (the C ABI declared in include/ptn_hip.h).
+
+Mirrors the reference's ALS interface for this path: options with the
+Options<Tensor> fields (PARTENSOR_basic.hpp:283-336), a status with the
+Status<Tensor> fields (PARTENSOR_basic.hpp:556-595), and cpd() / cpd_dimtree()
+taking a tensor, a rank, options and initial factors (Cpd.hpp:550-577,
+CpdDimTree.hpp:534-565).  Tensors are numpy arrays in Fortran order (mode 0
+fastest), factors (I_n, R) arrays.  There is no CPU fallback: if the HIP
+library is missing or no GPU is present the calls raise.
+"""
+import ctypes
+import math
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libptn_hip.so")
+MAX_ORDER = 8
+MAX_RANK = 64
+
+UNCONSTRAINED, NONNEGATIVITY, ORTHOGONALITY, SPARSITY, CONSTANT = 0, 1, 2, 3, 4
+TENSOR_F64, TENSOR_F32, TENSOR_BOTH = 1, 2, 3
+MTTKRP_STANDARD, MTTKRP_DIMTREE = 0, 2
+ARITH_F64, ARITH_F32, ARITH_BF16X3, ARITH_F32_F64 = 0, 1, 2, 3
+
+EXPORTED = [
+    "ptn_create", "ptn_destroy", "ptn_last_error", "ptn_default_options", "ptn_abi_version",
+    "ptn_set_tensor", "ptn_generate_tensor", "ptn_generate_factors", "ptn_tensor_norm_sq",
+    "ptn_cpd", "ptn_mttkrp", "ptn_mttkrp_dimtree", "ptn_nesterov_mnls", "ptn_sym_extremes",
+    "ptn_normalize", "ptn_cost_explicit", "ptn_time_contraction",
+]
+
+_d, _i, _i64, _u64 = ctypes.c_double, ctypes.c_int, ctypes.c_int64, ctypes.c_uint64
+_pd = ctypes.POINTER(ctypes.c_double)
+_pi64 = ctypes.POINTER(ctypes.c_int64)
+_vp = ctypes.c_void_p
+
+
+class Options(ctypes.Structure):
+    _fields_ = [
+        ("max_iter", _i), ("threshold_error", _d), ("nesterov_delta_1", _d), ("nesterov_delta_2", _d),
+        ("accel_coeff", _i), ("accel_fail", _i), ("acceleration", _i), ("normalization", _i),
+        ("constraints", _i * MAX_ORDER), ("mttkrp_form", _i), ("arith", _i), ("max_inner", _i),
+        ("fuse_accel", _i),
+    ]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("frob_sq", _d), ("f_value", _d), ("rel_cost", _d), ("ao_iter", _i), ("accel_coeff", _i),
+                ("accel_fail", _i), ("sweep_seconds", _d)]
+
+
+class SweepLog(ctypes.Structure):
+    _fields_ = [
+        ("f_value", _d), ("rel_cost", _d), ("f_accel", _d), ("accel_tried", _i), ("accel_accepted", _i),
+        ("nnls_iters", _i * MAX_ORDER), ("zero_col_fallback", _i * MAX_ORDER),
+        ("lipschitz", _d * MAX_ORDER), ("mu", _d * MAX_ORDER),
+    ]
+
+
+class PtnError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("ptn status %d: %s" % (code, msg))
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load libptn_hip.so; raises if it has not been built (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libptn_hip.so is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        L.ptn_create.argtypes = [ctypes.POINTER(_vp), _i]
+        L.ptn_destroy.argtypes = [_vp]
+        L.ptn_destroy.restype = None
+        L.ptn_last_error.argtypes = [_vp]
+        L.ptn_last_error.restype = ctypes.c_char_p
+        L.ptn_default_options.argtypes = [ctypes.POINTER(Options)]
+        L.ptn_default_options.restype = None
+        L.ptn_set_tensor.argtypes = [_vp, _i, _pi64, _pd, _i]
+        L.ptn_generate_tensor.argtypes = [_vp, _i, _pi64, _i, _u64, _i, _d, _i]
+        L.ptn_generate_factors.argtypes = [_vp, _i, _pi64, _i, _u64, _i, _pd]
+        L.ptn_tensor_norm_sq.argtypes = [_vp, _pd]
+        L.ptn_cpd.argtypes = [_vp, _i, _pd, ctypes.POINTER(Options), _pd, ctypes.POINTER(Result),
+                              ctypes.POINTER(SweepLog), _i]
+        L.ptn_mttkrp.argtypes = [_vp, _i, _pd, _i, _i, _pd]
+        L.ptn_mttkrp_dimtree.argtypes = [_vp, _i, _pd, _i, _pd]
+        L.ptn_nesterov_mnls.argtypes = [_vp, _i64, _i, _pd, _pd, _d, _d, _i, _pd, ctypes.POINTER(_i), _pd, _pd]
+        L.ptn_sym_extremes.argtypes = [_vp, _i, _pd, _pd, _pd]
+        L.ptn_normalize.argtypes = [_vp, _i, _pi64, _i, _i, _pd, _pd]
+        L.ptn_cost_explicit.argtypes = [_vp, _i, _pd, _pd]
+        L.ptn_time_contraction.argtypes = [_vp, _i, _i, _pd, _i, _i, _pd, _pd]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_pd)
+
+
+def _dims(dims):
+    return (ctypes.c_int64 * len(dims))(*[int(x) for x in dims])
+
+
+def pack(factors):
+    return np.ascontiguousarray(np.concatenate([np.asarray(f, dtype=np.float64).ravel(order="F") for f in factors]))
+
+
+def unpack(packed, dims, R):
+    out, o = [], 0
+    for I in dims:
+        out.append(np.array(packed[o:o + I * R].reshape((I, R), order="F")))
+        o += I * R
+    return out
+
+
+def default_options(**kw):
+    o = Options()
+    lib().ptn_default_options(ctypes.byref(o))
+    for k, v in kw.items():
+        if k == "constraints":
+            for i, c in enumerate(v):
+                o.constraints[i] = int(c)
+        else:
+            setattr(o, k, v)
+    return o
+
+
+class Handle:
+    """Owns one device context (streams, tensor, work buffers) through the C ABI."""
+
+    def __init__(self, device=0):
+        self._h = _vp()
+        rc = lib().ptn_create(ctypes.byref(self._h), device)
+        if rc != 0:
+            raise PtnError(rc, "ptn_create failed (device %d)" % device)
+        self.dims = None
+
+    def close(self):
+        if self._h:
+            lib().ptn_destroy(self._h)
+            self._h = _vp()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise PtnError(rc, lib().ptn_last_error(self._h).decode())
+
+    # -- tensor --
+    def set_tensor(self, X, storage=TENSOR_F64):
+        X = np.asarray(X, dtype=np.float64)
+        flat = np.ascontiguousarray(X.ravel(order="F"))
+        self._check(lib().ptn_set_tensor(self._h, X.ndim, _dims(X.shape), _p(flat), storage))
+        self.dims = list(X.shape)
+
+    def generate_tensor(self, dims, R_true, seed, snr_db=None, dist=0, storage=TENSOR_F64):
+        snr = float("nan") if snr_db is None else float(snr_db)
+        self._check(lib().ptn_generate_tensor(self._h, len(dims), _dims(dims), R_true, seed, dist, snr, storage))
+        self.dims = list(dims)
+
+    def generate_factors(self, dims, R, seed, dist=0):
+        out = np.empty(sum(dims) * R, dtype=np.float64)
+        self._check(lib().ptn_generate_factors(self._h, len(dims), _dims(dims), R, seed, dist, _p(out)))
+        return unpack(out, dims, R)
+
+    def norm_sq(self):
+        v = _d()
+        self._check(lib().ptn_tensor_norm_sq(self._h, ctypes.byref(v)))
+        return v.value
+
+    # -- ALS --
+    def cpd(self, R, init, options=None, **kw):
+        """cpd(tnsX, R, options, factorsInit) on the handle's tensor.  Returns
+        (factors, status dict, per-sweep log list)."""
+        o = options if options is not None else default_options(**kw)
+        dims = self.dims
+        out = np.empty(sum(dims) * R, dtype=np.float64)
+        res = Result()
+        cap = max(int(o.max_iter), 1)
+        logs = (SweepLog * cap)()
+        self._check(lib().ptn_cpd(self._h, R, _p(pack(init)), ctypes.byref(o), _p(out), ctypes.byref(res), logs,
+                                  cap))
+        N = len(dims)
+        status = {"frob_sq": res.frob_sq, "f_value": res.f_value, "rel_cost": res.rel_cost, "ao_iter": res.ao_iter,
+                  "accel_coeff": res.accel_coeff, "accel_fail": res.accel_fail,
+                  "sweep_seconds": res.sweep_seconds}
+        log = []
+        for i in range(min(res.ao_iter, cap)):
+            r = logs[i]
+            log.append({"f_value": r.f_value, "rel_cost": r.rel_cost, "f_accel": r.f_accel,
+                        "accel_tried": r.accel_tried, "accel_accepted": r.accel_accepted,
+                        "nnls_iters": list(r.nnls_iters[:N]), "zero_col_fallback": list(r.zero_col_fallback[:N]),
+                        "lipschitz": list(r.lipschitz[:N]), "mu": list(r.mu[:N])})
+        return unpack(out, dims, R), status, log
+
+    def cpd_dimtree(self, R, init, **kw):
+        kw.setdefault("mttkrp_form", MTTKRP_DIMTREE)
+        return self.cpd(R, init, **kw)
+
+    # -- single operations --
+    def mttkrp(self, factors, mode, arith=ARITH_F64):
+        R = factors[0].shape[1]
+        rows = self.dims[mode] if 0 <= mode < len(self.dims) else 1   # the C ABI rejects a bad mode
+        M = np.empty(rows * R, dtype=np.float64)
+        self._check(lib().ptn_mttkrp(self._h, R, _p(pack(factors)), mode, arith, _p(M)))
+        return M.reshape((rows, R), order="F")
+
+    def mttkrp_dimtree(self, factors, arith=ARITH_F64):
+        R = factors[0].shape[1]
+        out = np.empty(sum(self.dims) * R, dtype=np.float64)
+        self._check(lib().ptn_mttkrp_dimtree(self._h, R, _p(pack(factors)), arith, _p(out)))
+        return unpack(out, self.dims, R)
+
+    def nesterov(self, Z, W, A0, d1=1e-2, d2=1e-2, max_inner=50):
+        Z = np.asfortranarray(Z, dtype=np.float64)
+        W = np.asfortranarray(W, dtype=np.float64)
+        A = np.array(A0, dtype=np.float64, order="F")
+        it, L, mu = _i(), _d(), _d()
+        self._check(lib().ptn_nesterov_mnls(self._h, A.shape[0], A.shape[1], _p(Z), _p(W), d1, d2, max_inner, _p(A),
+                                            ctypes.byref(it), ctypes.byref(L), ctypes.byref(mu)))
+        return A, it.value, L.value, mu.value
+
+    def sym_extremes(self, V):
+        V = np.asfortranarray(V, dtype=np.float64)
+        L, mu = _d(), _d()
+        self._check(lib().ptn_sym_extremes(self._h, V.shape[0], _p(V), ctypes.byref(L), ctypes.byref(mu)))
+        return L.value, mu.value
+
+    def normalize(self, grams, factors, weight_factor=0):
+        dims = [f.shape[0] for f in factors]
+        R = factors[0].shape[1]
+        gp = np.ascontiguousarray(np.concatenate([np.asarray(g, dtype=np.float64).ravel(order="F") for g in grams]))
+        fp = pack(factors)
+        self._check(lib().ptn_normalize(self._h, len(dims), _dims(dims), R, weight_factor, _p(gp), _p(fp)))
+        gs = [np.array(gp[k * R * R:(k + 1) * R * R].reshape((R, R), order="F")) for k in range(len(dims))]
+        return gs, unpack(fp, dims, R)
+
+    def cost_explicit(self, factors):
+        R = factors[0].shape[1]
+        f = _d()
+        self._check(lib().ptn_cost_explicit(self._h, R, _p(pack(factors)), ctypes.byref(f)))
+        return f.value
+
+    def time_contraction(self, kernel, factors, arith=ARITH_F64, iters=10):
+        R = factors[0].shape[1]
+        ms, b = _d(), _d()
+        self._check(lib().ptn_time_contraction(self._h, kernel, R, _p(pack(factors)), arith, iters,
+                                               ctypes.byref(ms), ctypes.byref(b)))
+        return ms.value, b.value
+
+
+def gpu_available():
+    """True when a HIP device is visible (counts devices without initialising one)."""
+    try:
+        import torch
+        return torch.cuda.device_count() > 0
+    except Exception:
+        return False
+
+
+def rel_err(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    den = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (den if den > 0 else 1.0))
+
+
+__all__ = ["Handle", "default_options", "pack", "unpack", "lib", "PtnError", "EXPORTED", "rel_err", "math"]

@@ -1,0 +1,235 @@
+"""GPU parity: the HIP path through the C ABI against the FP64 CPU oracle on the
+same inputs.  Integer-valued golden data must match exactly; FP64 runs within
+the tolerances written in each test.  Full-size (800^3) checks use properties:
+the closed form of an exact low-rank MTTKRP."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as o
+import ptn_hip as P
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-test]", *a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def h():
+    say("create handle")
+    with P.Handle(0) as hd:
+        yield hd
+
+
+# ------------------------------------------------------------------ MTTKRP --
+
+@pytest.mark.parametrize("name", ["mttkrp_3x4x5_r2.npz", "mttkrp_2x3x4x5_r3.npz"])
+def test_mttkrp_golden_exact(h, golden_dir, name):
+    say("golden", name)
+    g = np.load(os.path.join(golden_dir, name))
+    dims = [int(d) for d in g["dims"]]
+    X = g["X"].reshape(dims, order="F")
+    A = [g["A%d" % k] for k in range(len(dims))]
+    h.set_tensor(X)
+    for mode in range(len(dims)):
+        assert np.array_equal(h.mttkrp(A, mode), g["M%d" % mode]), mode
+    if len(dims) in (3, 4):
+        D = h.mttkrp_dimtree(A)
+        for mode in range(len(dims)):
+            assert np.array_equal(D[mode], g["M%d" % mode]), mode
+
+
+@pytest.mark.parametrize("dims,R", [([100, 100, 100], 10), ([37, 23, 41], 5), ([17, 65, 33], 33),
+                                    ([64, 16, 48], 64), ([9, 11, 13, 7], 17), ([3, 5, 4, 2, 3], 6)])
+def test_mttkrp_vs_oracle(h, dims, R):
+    say("mttkrp", dims, R)
+    rng = np.random.default_rng(sum(dims) + R)
+    X = np.asfortranarray(rng.standard_normal(dims))
+    A = [rng.random((I, R)) for I in dims]
+    h.set_tensor(X, storage=P.TENSOR_BOTH)
+    for mode in range(len(dims)):
+        ref = o.mttkrp(X, A, mode)
+        got = h.mttkrp(A, mode)
+        assert P.rel_err(got, ref) < 1e-13, (mode, P.rel_err(got, ref))
+        # FP32 tensor stream with FP64 matrix cores: bounded by the FP32 rounding of X
+        got32 = h.mttkrp(A, mode, arith=P.ARITH_F32_F64)
+        assert P.rel_err(got32, ref) < 1e-6
+    if len(dims) in (3, 4):
+        D = h.mttkrp_dimtree(A)
+        for mode in range(len(dims)):
+            assert P.rel_err(D[mode], o.mttkrp(X, A, mode)) < 1e-13, mode
+
+
+def test_mttkrp_full_size_closed_form(h):
+    """800^3 exact rank-32 tensor: M_n = B_n (Hadamard_{k!=n} B_k^T A_k)."""
+    dims, Rt, R = [800, 800, 800], 32, 32
+    say("full size closed form", dims)
+    h.generate_tensor(dims, Rt, seed=0, snr_db=None, storage=P.TENSOR_BOTH)
+    B = o.gen_factors(dims, Rt, seed=0)
+    A = o.gen_factors(dims, R, seed=1)
+    for mode in range(3):
+        H = np.ones((Rt, R))
+        for k in range(3):
+            if k != mode:
+                H *= B[k].T @ A[k]
+        ref = B[mode] @ H
+        assert P.rel_err(h.mttkrp(A, mode), ref) < 1e-12, mode
+    D = h.mttkrp_dimtree(A)
+    for mode in range(3):
+        H = np.ones((Rt, R))
+        for k in range(3):
+            if k != mode:
+                H *= B[k].T @ A[k]
+        assert P.rel_err(D[mode], B[mode] @ H) < 1e-12, mode
+
+
+# --------------------------------------------------------- generator / cost --
+
+def test_generator_matches_oracle(h):
+    say("generator")
+    dims = [30, 20, 10]
+    A_gpu = h.generate_factors(dims, 7, seed=5)
+    A_cpu = o.gen_factors(dims, 7, seed=5)
+    for a, b in zip(A_gpu, A_cpu):
+        assert np.array_equal(a, b)                       # integer arithmetic + exact conversion
+    h.generate_tensor(dims, 4, seed=3, snr_db=None)
+    X0, _ = o.make_tensor(dims, 4, seed=3)
+    assert h.norm_sq() == pytest.approx(np.sum(X0 ** 2), rel=1e-14)
+    h.generate_tensor(dims, 4, seed=3, snr_db=20.0)
+    Xn, _ = o.make_tensor(dims, 4, seed=3, snr_db=20.0)
+    assert h.norm_sq() == pytest.approx(np.sum(Xn ** 2), rel=1e-12)
+
+
+def test_cost_explicit_matches_oracle(h):
+    say("explicit cost")
+    X, At = o.make_tensor([40, 30, 20], 5, seed=0, snr_db=25.0)
+    init = o.gen_factors([40, 30, 20], 5, seed=1)
+    h.set_tensor(X)
+    assert h.cost_explicit(init) == pytest.approx(o.cost_explicit(X, init), rel=1e-13)
+
+
+# ------------------------------------------------------------ R x R pieces --
+
+def test_sym_extremes_match_oracle(h, golden_dir):
+    g = np.load(os.path.join(golden_dir, "eig_extremes.npz"))
+    for i in range(int(g["count"])):
+        V = g["V%d" % i]
+        say("eig", V.shape)
+        L, mu = h.sym_extremes(V)
+        Lo, muo, _ = o.sym_extremes(V)
+        assert L == Lo and mu == muo                       # same arithmetic sequence: bitwise
+
+
+@pytest.mark.parametrize("I,R,d", [(7, 4, 1e-2), (300, 32, 1e-2), (129, 64, 1e-2), (50, 10, 0.0), (20, 5, 1e30)])
+def test_nesterov_matches_oracle(h, I, R, d):
+    say("nnls", I, R, d)
+    rng = np.random.default_rng(I + R)
+    B = rng.random((3 * R, R))
+    Z = B.T @ B
+    W = rng.standard_normal((I, R)) * 4 + 2
+    A0 = rng.random((I, R))
+    Ag, itg, Lg, mug = h.nesterov(Z, W, A0, d, d, 50)
+    Ao, ito, Lo, muo, = o.nesterov(Z, W, A0, d, d, 50)
+    assert itg == ito
+    assert (Lg, mug) == (Lo, muo)
+    assert np.array_equal(Ag, Ao)                          # row arithmetic mirrors the oracle
+
+
+def test_normalize_matches_oracle(h):
+    rng = np.random.default_rng(3)
+    for N in (3, 4):
+        dims = [7, 6, 5, 4][:N]
+        A = [rng.random((I, 5)) for I in dims]
+        A[-1][:, 2] = 0.0                                  # zero-norm branch on one column
+        G = [a.T @ a for a in A]
+        say("normalize N=%d" % N)
+        Gg, Ag = h.normalize(G, A, 0)
+        Go, Ao = o.normalize(G, A, 0)
+        for a, b in zip(Ag, Ao):
+            assert np.array_equal(a, b)
+        for a, b in zip(Gg, Go):
+            assert np.array_equal(a, b)
+
+
+# --------------------------------------------------------------------- ALS --
+
+def _compare_runs(Fg, stg, logg, Fo, sto, logo, tol):
+    assert stg["ao_iter"] == sto["ao_iter"]
+    assert [l["nnls_iters"] for l in logg] == [l["nnls_iters"] for l in logo]
+    assert [l["accel_accepted"] for l in logg] == [l["accel_accepted"] for l in logo]
+    assert [l["zero_col_fallback"] for l in logg] == [l["zero_col_fallback"] for l in logo]
+    assert (stg["accel_coeff"], stg["accel_fail"]) == (sto["accel_coeff"], sto["accel_fail"])
+    assert stg["f_value"] == pytest.approx(sto["f_value"], rel=tol)
+    for a, b in zip(Fg, Fo):
+        assert P.rel_err(a, b) < tol
+
+
+@pytest.mark.parametrize("form", [P.MTTKRP_STANDARD, P.MTTKRP_DIMTREE])
+def test_als_c1_parity(h, form):
+    """Config 1: 100^3, R=10, SNR 30 dB, init seed 1, 50 sweeps, reference defaults."""
+    say("ALS C1 form", form)
+    dims, R = [100, 100, 100], 10
+    X, _ = o.make_tensor(dims, R, seed=0, snr_db=30.0)
+    init = o.gen_factors(dims, R, seed=1)
+    h.set_tensor(X)
+    kw = dict(max_iter=50, threshold_error=0.0)
+    Fg, stg, logg = h.cpd(R, init, mttkrp_form=form, **kw)
+    Fo, sto, logo = o.cpd(X, R, init, mttkrp_form=(o.MTTKRP_DIMTREE if form == P.MTTKRP_DIMTREE else
+                                                     o.MTTKRP_STANDARD), cost_form=o.COST_GRAM, **kw)
+    say("rel", stg["rel_cost"], sto["rel_cost"], "sweeps/s", stg["ao_iter"] / stg["sweep_seconds"])
+    _compare_runs(Fg, stg, logg, Fo, sto, logo, 1e-10)
+
+
+def test_als_c1_no_accel_no_norm(h):
+    say("ALS C1 variant: acceleration and normalization off")
+    dims, R = [100, 100, 100], 10
+    X, _ = o.make_tensor(dims, R, seed=0, snr_db=30.0)
+    init = o.gen_factors(dims, R, seed=1)
+    h.set_tensor(X)
+    kw = dict(max_iter=50, threshold_error=0.0, acceleration=0, normalization=0)
+    Fg, stg, logg = h.cpd(R, init, **kw)
+    Fo, sto, logo = o.cpd(X, R, init, cost_form=o.COST_GRAM, **kw)
+    _compare_runs(Fg, stg, logg, Fo, sto, logo, 1e-10)
+
+
+@pytest.mark.parametrize("dims,R", [([12, 9, 10, 8], 4), ([30, 31, 32], 33)])
+def test_als_small_parity(h, dims, R):
+    say("ALS small", dims, R)
+    X, _ = o.make_tensor(dims, min(R, 6), seed=2, snr_db=25.0)
+    init = o.gen_factors(dims, R, seed=3)
+    h.set_tensor(X)
+    kw = dict(max_iter=15, threshold_error=0.0)
+    for form, oform in ((P.MTTKRP_STANDARD, o.MTTKRP_STANDARD), (P.MTTKRP_DIMTREE, o.MTTKRP_DIMTREE)):
+        Fg, stg, logg = h.cpd(R, init, mttkrp_form=form, **kw)
+        Fo, sto, logo = o.cpd(X, R, init, mttkrp_form=oform, cost_form=o.COST_GRAM, **kw)
+        _compare_runs(Fg, stg, logg, Fo, sto, logo, 1e-10)
+
+
+def test_zero_column_fallback_on_gpu(h):
+    say("zero-column fallback")
+    rng = np.random.default_rng(12)
+    dims = [6, 7, 8]
+    X = -o.cp_full([rng.random((I, 2)) for I in dims])
+    init = [rng.random((I, 3)) for I in dims]
+    h.set_tensor(X)
+    kw = dict(max_iter=1, threshold_error=0.0, normalization=0, acceleration=0)
+    Fg, stg, logg = h.cpd(3, init, **kw)
+    Fo, sto, logo = o.cpd(X, 3, init, cost_form=o.COST_GRAM, **kw)
+    assert logg[0]["zero_col_fallback"][0] == 1
+    assert np.array_equal(Fg[0], Fo[0])
+
+
+def test_invalid_arguments_raise(h):
+    say("error paths")
+    h.set_tensor(np.ones((4, 5, 6), order="F"))
+    init = [np.ones((I, 3)) for I in (4, 5, 6)]
+    with pytest.raises(P.PtnError):
+        h.cpd(65, [np.ones((I, 65)) for I in (4, 5, 6)])          # rank above the wave-per-row limit
+    with pytest.raises(P.PtnError):
+        h.mttkrp(init, 5)                                           # bad mode
+    with pytest.raises(P.PtnError):
+        h.mttkrp(init, 0, arith=P.ARITH_F32_F64)                    # no FP32 copy uploaded
