@@ -41,7 +41,8 @@ enum { PTN_MTTKRP_STANDARD = 0, PTN_MTTKRP_DIMTREE = 2 };
 /* Arithmetic of the tensor contractions. */
 enum { PTN_ARITH_F64 = 0,      /* FP64 tensor, FP64 matrix cores (the plain path)        */
        PTN_ARITH_F32 = 1,      /* FP32 tensor, FP32 matrix cores, FP32 accumulate        */
-       PTN_ARITH_BF16X3 = 2 }; /* FP32 tensor, bf16x3 split products, FP32 accumulate    */
+       PTN_ARITH_BF16X3 = 2,   /* FP32 tensor, bf16x3 split products, FP32 accumulate    */
+       PTN_ARITH_F32_F64 = 3 };/* FP32 tensor, FP64 matrix cores, FP64 accumulate        */
 
 /* Options<Tensor> fields the ALS path consumes (PARTENSOR_basic.hpp:283-336) plus
    the execution choices of this policy. */
@@ -82,6 +83,7 @@ typedef struct {
   int    zero_col_fallback[PTN_MAX_ORDER];
   double lipschitz[PTN_MAX_ORDER];
   double mu[PTN_MAX_ORDER];
+  double t_end;        /* seconds from the start of the outer loop to the end of this sweep (device-synchronised) */
 } ptn_sweep_log;
 
 /* ---- handle ---- */
@@ -129,8 +131,10 @@ ptn_status ptn_cost_explicit(ptn_handle h, int R, const double *factors_packed, 
 
 /* Average device time (ms) of `iters` back-to-back launches of one contraction
    kernel on the handle's stream, bracketed by HIP events on that stream.
-   kernel: 0 = left node X x_{N-1} A_{N-1} (gemm_cm), 1 = right node X x_0 A_0 (gemm_ck),
-   2 = mode-0 standard MTTKRP.  bytes_out: algorithmic bytes of one launch. */
+   kernel: 0 = left node X x_{N-1} A_{N-1} (gemm_cm), 1 = right node X x_0 A_0 (gemm_ck; for an
+   order-3 tensor on the FP32-output paths fused with the TTV by A_1, i.e. the mode-2 MTTKRP),
+   2 = mode-0 standard MTTKRP.  Operand packing is done before the timed launches.
+   bytes_out: algorithmic bytes of one launch (tensor once + output + factor rows). */
 ptn_status ptn_time_contraction(ptn_handle h, int kernel, int R, const double *factors_packed, int arith, int iters,
                                 double *avg_ms, double *bytes_out);
 

@@ -48,7 +48,7 @@ class SweepLog(ctypes.Structure):
     _fields_ = [
         ("f_value", _d), ("rel_cost", _d), ("f_accel", _d), ("accel_tried", _i), ("accel_accepted", _i),
         ("nnls_iters", _i * MAX_ORDER), ("zero_col_fallback", _i * MAX_ORDER),
-        ("lipschitz", _d * MAX_ORDER), ("mu", _d * MAX_ORDER),
+        ("lipschitz", _d * MAX_ORDER), ("mu", _d * MAX_ORDER), ("t_start", _d), ("t_end", _d),
     ]
 
 
@@ -312,6 +312,7 @@ def _log_to_dict(rec, N):
         "accel_tried": rec.accel_tried, "accel_accepted": rec.accel_accepted,
         "nnls_iters": list(rec.nnls_iters[:N]), "zero_col_fallback": list(rec.zero_col_fallback[:N]),
         "lipschitz": list(rec.lipschitz[:N]), "mu": list(rec.mu[:N]),
+        "t_start": rec.t_start, "t_end": rec.t_end,
     }
 
 

@@ -15,6 +15,15 @@
 
 #define PTNO_DBL_EPS 2.220446049250313e-16
 
+static double wall_seconds(void)
+{
+#ifdef _OPENMP
+  return omp_get_wtime();
+#else
+  return 0.0;
+#endif
+}
+
 /* ------------------------------------------------------------------------- */
 /* options                                                                    */
 /* ------------------------------------------------------------------------- */
@@ -327,14 +336,19 @@ void ptno_mttkrp_blocked(int N, const int64_t *dims, int R, const double *X, con
 /* small dense pieces                                                         */
 /* ------------------------------------------------------------------------- */
 
-/* G = A^T A (Cpd.hpp:227), sum over rows in increasing order. */
+/* G = A^T A (Cpd.hpp:227): rows summed in order within chunks of 32, then the
+   chunk sums added in order. */
 void ptno_gram(int64_t I, int R, const double *A, double *G)
 {
   for (int a = 0; a < R; ++a)
     for (int b = 0; b < R; ++b) {
-      double s = 0.0;
-      for (int64_t i = 0; i < I; ++i) s = fma(A[i + I * a], A[i + I * b], s);
-      G[a + R * b] = s;
+      double g = 0.0;
+      for (int64_t c = 0; c < I; c += 32) {
+        double s = 0.0;
+        for (int64_t i = c; i < I && i < c + 32; ++i) s = fma(A[i + I * a], A[i + I * b], s);
+        g = c == 0 ? s : g + s;
+      }
+      G[a + R * b] = g;
     }
 }
 
@@ -424,22 +438,125 @@ static int jacobi_rr(int R, double *a, double *v)
   return sweep;
 }
 
+/* ---- extreme eigenvalues: Householder tridiagonalization + Sturm multisection ----
+   Golub & Van Loan, Matrix Computations 4th ed., Alg. 8.3.1 (tridiagonalization
+   by Householder reflections, v = x + sign(x0)||x|| e1) and 8.4.1 (Sturm sequence
+   bisection, here 31 probe points per step).  Every sum runs in index order with
+   fma; the device kernel performs the same operations in the same order. */
+
+#define PTNO_PIVMIN_BASE 2.2250738585072014e-308
+
+static void tridiagonalize(int n, double *a /* row-major, destroyed */, double *d, double *e)
+{
+  double *v = (double *)malloc(sizeof(double) * n), *p = (double *)malloc(sizeof(double) * n);
+  for (int k = 0; k + 2 < n; ++k) {
+    double sigma = 0.0;
+    for (int i = k + 2; i < n; ++i) sigma = fma(a[i * n + k], a[i * n + k], sigma);
+    double x0 = a[(k + 1) * n + k];
+    if (sigma == 0.0) { e[k] = x0; continue; }
+    double mu = sqrt(fma(x0, x0, sigma));
+    double v0 = x0 <= 0.0 ? x0 - mu : x0 + mu;
+    e[k] = x0 <= 0.0 ? mu : -mu;
+    double vtv = fma(v0, v0, sigma);
+    double beta = 2.0 / vtv;
+    v[k + 1] = v0;
+    for (int i = k + 2; i < n; ++i) v[i] = a[i * n + k];
+    for (int i = k + 1; i < n; ++i) {
+      double s = 0.0;
+      for (int j = k + 1; j < n; ++j) s = fma(a[i * n + j], v[j], s);
+      p[i] = beta * s;
+    }
+    double t = 0.0;
+    for (int i = k + 1; i < n; ++i) t = fma(p[i], v[i], t);
+    double K = (0.5 * beta) * t;
+    for (int i = k + 1; i < n; ++i) p[i] = fma(-K, v[i], p[i]);          /* w = p - K v */
+    for (int i = k + 1; i < n; ++i)
+      for (int j = k + 1; j < n; ++j) a[i * n + j] = a[i * n + j] - (v[i] * p[j] + p[i] * v[j]);
+  }
+  for (int i = 0; i < n; ++i) d[i] = a[i * n + i];
+  if (n >= 2) e[n - 2] = a[(n - 1) * n + (n - 2)];
+  free(v); free(p);
+}
+
+/* number of eigenvalues of the tridiagonal (d, e2 = e^2) below x */
+static int sturm_count(int n, const double *d, const double *e2, double pivmin, double x)
+{
+  int c = 0;
+  double q = d[0] - x;
+  if (fabs(q) < pivmin) q = -pivmin;
+  if (q < 0.0) ++c;
+  for (int i = 1; i < n; ++i) {
+    q = (d[i] - x) - e2[i - 1] / q;
+    if (fabs(q) < pivmin) q = -pivmin;
+    if (q < 0.0) ++c;
+  }
+  return c;
+}
+
+/* j-th smallest eigenvalue (0-based) by multisection on [lo, hi], count(lo) <= j < count(hi) */
+static double sturm_find(int n, const double *d, const double *e2, double pivmin, double lo, double hi, int j)
+{
+  for (int it = 0; it < 64; ++it) {
+    double w = hi - lo;
+    double sc = fabs(lo) > fabs(hi) ? fabs(lo) : fabs(hi);
+    if (w <= 2.0 * PTNO_DBL_EPS * sc || w <= pivmin) break;
+    int m;
+    for (m = 0; m < 31; ++m) {
+      double x = fma(w, (double)(m + 1) * 0.03125, lo);
+      if (sturm_count(n, d, e2, pivmin, x) > j) break;
+    }
+    double nlo = m > 0 ? fma(w, (double)m * 0.03125, lo) : lo;
+    double nhi = m < 31 ? fma(w, (double)(m + 1) * 0.03125, lo) : hi;
+    if (nlo == lo && nhi == hi) break;
+    lo = nlo;
+    hi = nhi;
+  }
+  return 0.5 * (lo + hi);
+}
+
 /* Extreme singular values of the symmetric V (ComputeSVD, NesterovMNLS.hpp:45-52):
-   for a symmetric matrix they are max |lambda| and min |lambda|. */
+   for a symmetric matrix they are max |lambda| and min |lambda|.  Returns the
+   number of bisection targets evaluated (2 or 4). */
 int ptno_sym_extremes(int R, const double *V, double *L, double *mu)
 {
-  double *a = (double *)malloc((size_t)R * R * sizeof(double));
-  memcpy(a, V, (size_t)R * R * sizeof(double));
-  int sweeps = jacobi_rr(R, a, NULL);
-  double mx = fabs(a[0]), mn = fabs(a[0]);
-  for (int i = 1; i < R; ++i) {
-    double d = fabs(a[i + R * i]);
-    if (d > mx) mx = d;
-    if (d < mn) mn = d;
+  const int n = R;
+  double *a = (double *)malloc((size_t)n * n * sizeof(double));
+  double *d = (double *)malloc(sizeof(double) * n), *e = (double *)calloc((size_t)n, sizeof(double));
+  double *e2 = (double *)calloc((size_t)n, sizeof(double));
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) a[i * n + j] = V[i + (int64_t)n * j];
+  tridiagonalize(n, a, d, e);
+  double emax = 1.0;
+  for (int i = 0; i + 1 < n; ++i) { e2[i] = e[i] * e[i]; if (e2[i] > emax) emax = e2[i]; }
+  double pivmin = PTNO_PIVMIN_BASE * emax;
+  double glo = d[0], ghi = d[0];
+  for (int i = 0; i < n; ++i) {
+    double r = (i > 0 ? fabs(e[i - 1]) : 0.0) + (i + 1 < n ? fabs(e[i]) : 0.0);
+    double lo = d[i] - r, hi = d[i] + r;
+    if (lo < glo) glo = lo;
+    if (hi > ghi) ghi = hi;
   }
-  *L = mx; *mu = mn;
-  free(a);
-  return sweeps;
+  double pad = 2.0 * PTNO_DBL_EPS * (fabs(glo) > fabs(ghi) ? fabs(glo) : fabs(ghi)) + pivmin;
+  glo -= pad;
+  ghi += pad;
+  double lmin = sturm_find(n, d, e2, pivmin, glo, ghi, 0);
+  double lmax = sturm_find(n, d, e2, pivmin, glo, ghi, n - 1);
+  int k0 = sturm_count(n, d, e2, pivmin, 0.0);
+  int targets = 2;
+  double amin;
+  if (k0 == 0) amin = fabs(lmin);
+  else if (k0 == n) amin = fabs(lmax);
+  else {
+    double a1 = fabs(sturm_find(n, d, e2, pivmin, glo, ghi, k0 - 1));
+    double a2 = fabs(sturm_find(n, d, e2, pivmin, glo, ghi, k0));
+    amin = a1 < a2 ? a1 : a2;
+    targets = 4;
+  }
+  double amax = fabs(lmin) > fabs(lmax) ? fabs(lmin) : fabs(lmax);
+  *L = amax;
+  *mu = amin;
+  free(a); free(d); free(e); free(e2);
+  return targets;
 }
 
 int ptno_sym_eig(int R, const double *V, double *evals, double *evecs)
@@ -505,8 +622,10 @@ int ptno_nesterov_mnls(int64_t I, int R, const double *Z, const double *W, doubl
     double gmax = 0.0, gmin = INFINITY;
     for (int64_t i = 0; i < I; ++i)
       for (int r = 0; r < R; ++r) {
-        double s = 0.0;
-        for (int k = 0; k < R; ++k) s = fma(Y[i + I * k], Zt[k + R * r], s);
+        /* (Y Zt)(i, r): four interleaved partial sums over k mod 4, combined pairwise */
+        double s4[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int k = 0; k < R; ++k) s4[k & 3] = fma(Y[i + I * k], Zt[k + R * r], s4[k & 3]);
+        double s = (s4[0] + s4[1]) + (s4[2] + s4[3]);
         double g = s - Wt[i + I * r];                                      /* grad = Y Zt - Wt */
         G[i + I * r] = g;
         double gy = fabs(g * Y[i + I * r]);
@@ -963,6 +1082,7 @@ int ptno_cpd(int N, const int64_t *dims, int R, const double *X, const double *i
     st->ao_iter++;
     ptno_sweep_log lrec;
     memset(&lrec, 0, sizeof(lrec));
+    lrec.t_start = wall_seconds();
     for (int i = 0; i < N; ++i) {
       if (use_tree) dimtree_mode(&tree, X, A, grams, i, M, V);
       else {
@@ -1012,6 +1132,7 @@ int ptno_cpd(int N, const int64_t *dims, int R, const double *X, const double *i
       }
       memcpy(old, nrm, sizeof(double) * fsz);
     }
+    lrec.t_end = wall_seconds();
     if (log && st->ao_iter - 1 < log_cap) log[st->ao_iter - 1] = lrec;
     if (stop) break;
   }

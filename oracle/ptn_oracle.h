@@ -72,6 +72,7 @@ typedef struct {
   int    zero_col_fallback[PTNO_MAX_ORDER];
   double lipschitz[PTNO_MAX_ORDER];   /* L and mu of V_n before GLambda */
   double mu[PTNO_MAX_ORDER];
+  double t_start, t_end;              /* wall seconds (omp_get_wtime) at the sweep's start and end */
 } ptno_sweep_log;
 
 void ptno_default_options(ptno_options *o);

@@ -66,33 +66,41 @@ void contract_rows(hipStream_t s, const double *T64, const float *T32, int64_t l
                    int outmode, int ldr, int splits, double *partials);
 void reduce_splits(hipStream_t s, const double *partials, int splits, int64_t n, double *out);
 
-// FP32-stream MFMA variants (bf16x3 split products or FP32 MFMA), FP32 outputs.
+// FP32-stream MFMA variants (bf16x3 split products or FP32 MFMA), FP32 outputs with
+// RPF = R rounded up to 32 columns.  The Khatri-Rao operand is packed once per
+// contraction by pack_w (bf16 hi/lo fragments for bf16x3, FP32 rows for FP32 MFMA).
 enum GemmMode { kGemmFp32 = 0, kGemmBf16x3 = 1 };
-void gemm_cm_f32(hipStream_t s, GemmMode mode, const float *x32, int64_t Mrows, int64_t Ktot,
-                 const KrSpec &w, int ldr, int splits, float *out);
-void gemm_ck_f32(hipStream_t s, GemmMode mode, const float *x32, int64_t L, int64_t C,
-                 const float *A0, int ldr, float *T);
+int64_t wpack_elems(int64_t K, int RPF);   // number of 16-byte fragments (per hi / lo array)
+// kperm 0: fragments for gemm_cm_f32 (16-wide k steps); kperm 1: for gemm_ck_f32 (32-wide)
+void pack_w(hipStream_t s, const KrSpec &w, int64_t K, int RPF, int ldr, int kperm, void *whi, void *wlo, float *wf);
+void gemm_cm_f32(hipStream_t s, GemmMode mode, const float *x32, int64_t M, int64_t K, int RPF, int splits,
+                 const void *whi, const void *wlo, const float *wf, float *out);
+// fuse_I1 > 0: C = I1 * I2 fibers, epilogue TTV with A1 (FP64 rows, ldr) into part[I2][ceil(I1/64)][ldr]
+// (reduce_chunks then gives the order-3 mode-2 MTTKRP); fuse_I1 == 0: writes T[c][RPF].
+void gemm_ck_f32(hipStream_t s, GemmMode mode, const float *x32, int64_t L, int64_t C, int RPF, const void *whi,
+                 const void *wlo, const float *wf, float *T, int64_t fuse_I1, const double *A1, int ldr,
+                 double *part);
+void reduce_chunks(hipStream_t s, const double *part, int64_t I2, int64_t wpf, int ldr, double *M);
+void reduce_splits_f32(hipStream_t s, const float *partials, int splits, int64_t n, float *out);
 
 // ---- small dense pieces (small_kernels.hip) ----
 void hadamard(hipStream_t s, int R, int ng, const double *const *grams, double *V);
-void gram_partials(hipStream_t s, int64_t I, int R, int ldr, const double *A, double *partials, int nparts);
-void gram_reduce(hipStream_t s, int R, const double *partials, int nparts, double *G);
+// partials: ceil(I/32) x R x R doubles
+void gram(hipStream_t s, int64_t I, int R, int ldr, const double *A, double *G, double *partials);
 void eig_extremes(hipStream_t s, int R, const double *V, NnlsParams *p);
-void nnls_init_mask(hipStream_t s, uint64_t *mask, double *colsum, int R);
-void nnls_pass1(hipStream_t s, int64_t I, int R, int ldr, const double *V, const NnlsParams *p,
-                const double *M, const double *A, double d1, double d2, int max_inner, uint64_t *mask);
-void nnls_pass2(hipStream_t s, int64_t I, int R, int ldr, const double *V, NnlsParams *p,
-                const double *M, const double *Aold, double *Anew, int max_inner, const uint64_t *mask,
-                double *colsum);
+// betas: max_inner doubles; resets mask and colsum
+void nnls_prep(hipStream_t s, const NnlsParams *p, int max_inner, double *betas, uint64_t *mask, double *colsum,
+               int R);
+// hist: (max_inner + 1) x I x ldr doubles
+void nnls_iterate(hipStream_t s, int64_t I, int R, int ldr, const double *V, const NnlsParams *p,
+                  const double *betas, const double *M, const double *A, double d1, double d2, int max_inner,
+                  uint64_t *mask, double *hist);
+void nnls_select(hipStream_t s, int64_t I, int R, int ldr, const double *hist, const uint64_t *mask, int max_inner,
+                 double *Anew, double *colsum, NnlsParams *p);
 void zero_col_fallback(hipStream_t s, int64_t I, int R, int ldr, const double *colsum, const double *Aold,
                        double *A, NnlsParams *p);
-void unconstrained_update(hipStream_t s, int64_t I, int R, int ldr, const double *V, const double *M, double *A,
-                          double *scratch);
-void dot_partials(hipStream_t s, int64_t n, const double *a, const double *b, double *partials, int nparts);
-void cost_gram_final(hipStream_t s, int R, const double *frob, const double *ma_partials, int nparts,
-                     const double *V, const double *G, double *f);
-void accel_cost_final(hipStream_t s, int R, const double *frob, const double *ma_partials, int nparts,
-                      const double *V, const double *G, double *f);
+void cost_gram(hipStream_t s, int64_t nma, const double *M, const double *A, int R, const double *V, const double *G,
+               const double *frob, double *f, int accel);
 void normalize(hipStream_t s, int N, const int64_t *dims, int R, int ldr, int wf, double *const *grams,
                double *const *fac, double *scratch);
 void axpy_accel(hipStream_t s, int64_t n, double step, const double *old, const double *cur, double *out);

@@ -55,7 +55,7 @@ class SweepLog(ctypes.Structure):
     _fields_ = [
         ("f_value", _d), ("rel_cost", _d), ("f_accel", _d), ("accel_tried", _i), ("accel_accepted", _i),
         ("nnls_iters", _i * MAX_ORDER), ("zero_col_fallback", _i * MAX_ORDER),
-        ("lipschitz", _d * MAX_ORDER), ("mu", _d * MAX_ORDER),
+        ("lipschitz", _d * MAX_ORDER), ("mu", _d * MAX_ORDER), ("t_end", _d),
     ]
 
 
@@ -206,7 +206,7 @@ class Handle:
             log.append({"f_value": r.f_value, "rel_cost": r.rel_cost, "f_accel": r.f_accel,
                         "accel_tried": r.accel_tried, "accel_accepted": r.accel_accepted,
                         "nnls_iters": list(r.nnls_iters[:N]), "zero_col_fallback": list(r.zero_col_fallback[:N]),
-                        "lipschitz": list(r.lipschitz[:N]), "mu": list(r.mu[:N])})
+                        "lipschitz": list(r.lipschitz[:N]), "mu": list(r.mu[:N]), "t_end": r.t_end})
         return unpack(out, dims, R), status, log
 
     def cpd_dimtree(self, R, init, **kw):

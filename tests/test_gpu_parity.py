@@ -58,10 +58,18 @@ def test_mttkrp_vs_oracle(h, dims, R):
         # FP32 tensor stream with FP64 matrix cores: bounded by the FP32 rounding of X
         got32 = h.mttkrp(A, mode, arith=P.ARITH_F32_F64)
         assert P.rel_err(got32, ref) < 1e-6
+        # FP32 matrix cores (FP32 products, FP32 accumulation) and bf16x3 split products
+        assert P.rel_err(h.mttkrp(A, mode, arith=P.ARITH_F32), ref) < 2e-5, mode
+        assert P.rel_err(h.mttkrp(A, mode, arith=P.ARITH_BF16X3), ref) < 2e-4, mode
     if len(dims) in (3, 4):
         D = h.mttkrp_dimtree(A)
+        D3 = h.mttkrp_dimtree(A, arith=P.ARITH_BF16X3)
+        D2 = h.mttkrp_dimtree(A, arith=P.ARITH_F32)
         for mode in range(len(dims)):
-            assert P.rel_err(D[mode], o.mttkrp(X, A, mode)) < 1e-13, mode
+            ref = o.mttkrp(X, A, mode)
+            assert P.rel_err(D[mode], ref) < 1e-13, mode
+            assert P.rel_err(D2[mode], ref) < 2e-5, mode
+            assert P.rel_err(D3[mode], ref) < 2e-4, mode
 
 
 def test_mttkrp_full_size_closed_form(h):
@@ -79,12 +87,18 @@ def test_mttkrp_full_size_closed_form(h):
         ref = B[mode] @ H
         assert P.rel_err(h.mttkrp(A, mode), ref) < 1e-12, mode
     D = h.mttkrp_dimtree(A)
+    D3 = h.mttkrp_dimtree(A, arith=P.ARITH_BF16X3)
+    D2 = h.mttkrp_dimtree(A, arith=P.ARITH_F32)
     for mode in range(3):
         H = np.ones((Rt, R))
         for k in range(3):
             if k != mode:
                 H *= B[k].T @ A[k]
-        assert P.rel_err(D[mode], B[mode] @ H) < 1e-12, mode
+        ref = B[mode] @ H
+        assert P.rel_err(D[mode], ref) < 1e-12, mode
+        say("full size", mode, "fp32 mfma err", P.rel_err(D2[mode], ref), "bf16x3 err", P.rel_err(D3[mode], ref))
+        assert P.rel_err(D2[mode], ref) < 1e-5, mode
+        assert P.rel_err(D3[mode], ref) < 1e-4, mode
 
 
 # --------------------------------------------------------- generator / cost --
