@@ -446,36 +446,60 @@ static int jacobi_rr(int R, double *a, double *v)
 
 #define PTNO_PIVMIN_BASE 2.2250738585072014e-308
 
-static void tridiagonalize(int n, double *a /* row-major, destroyed */, double *d, double *e)
+/* Balanced tree sum of x[0..w) over consecutive pairs, ((x0 + x1) + (x2 + x3)) + ...
+   (w a power of two; x is overwritten). */
+static double tree_sum(double *x, int w)
 {
-  double *v = (double *)malloc(sizeof(double) * n), *p = (double *)malloc(sizeof(double) * n);
+  for (int s = 1; s < w; s <<= 1)
+    for (int i = 0; i < w; i += 2 * s) x[i] = x[i] + x[i + s];
+  return x[0];
+}
+
+/* Householder tridiagonalization (Golub & Van Loan Alg. 8.3.1) of the symmetric
+   matrix stored in Vraw (n x n, element (r, c) read from Vraw[r * n + c]).  The
+   working matrix t is padded to RB = 16, 32 or 64 rows/columns of zeros; step k
+   eliminates row k of t, forms p_l = sum_j t[j][l] v_j with four interleaved partial
+   sums over all RB rows, and takes sigma and p.v as balanced tree sums over 64
+   entries.  This operation order is the GPU kernel's (one wave, lane l = index l),
+   so the two agree bit for bit; for n > 64 RB = n and the trees widen. */
+static void tridiagonalize(int n, const double *Vraw, double *d, double *e)
+{
+  const int RB = n <= 16 ? 16 : n <= 32 ? 32 : n <= 64 ? 64 : n;
+  int W = 64;
+  while (W < RB) W <<= 1;
+  double *t = (double *)calloc((size_t)RB * RB, sizeof(double));
+  double *v = (double *)calloc((size_t)W, sizeof(double)), *p = (double *)calloc((size_t)W, sizeof(double));
+  double *xs = (double *)calloc((size_t)W, sizeof(double)), *tmp = (double *)calloc((size_t)W, sizeof(double));
+  for (int r = 0; r < n; ++r)
+    for (int c = 0; c < n; ++c) t[r * RB + c] = Vraw[(int64_t)r * n + c];
   for (int k = 0; k + 2 < n; ++k) {
-    double sigma = 0.0;
-    for (int i = k + 2; i < n; ++i) sigma = fma(a[i * n + k], a[i * n + k], sigma);
-    double x0 = a[(k + 1) * n + k];
-    if (sigma == 0.0) { e[k] = x0; continue; }
-    double mu = sqrt(fma(x0, x0, sigma));
-    double v0 = x0 <= 0.0 ? x0 - mu : x0 + mu;
-    e[k] = x0 <= 0.0 ? mu : -mu;
-    double vtv = fma(v0, v0, sigma);
-    double beta = 2.0 / vtv;
-    v[k + 1] = v0;
-    for (int i = k + 2; i < n; ++i) v[i] = a[i * n + k];
-    for (int i = k + 1; i < n; ++i) {
-      double s = 0.0;
-      for (int j = k + 1; j < n; ++j) s = fma(a[i * n + j], v[j], s);
-      p[i] = beta * s;
+    for (int l = 0; l < W; ++l) {
+      xs[l] = (l >= k + 2 && l < n) ? t[k * RB + l] : 0.0;
+      tmp[l] = xs[l] * xs[l];
     }
-    double t = 0.0;
-    for (int i = k + 1; i < n; ++i) t = fma(p[i], v[i], t);
-    double K = (0.5 * beta) * t;
-    for (int i = k + 1; i < n; ++i) p[i] = fma(-K, v[i], p[i]);          /* w = p - K v */
-    for (int i = k + 1; i < n; ++i)
-      for (int j = k + 1; j < n; ++j) a[i * n + j] = a[i * n + j] - (v[i] * p[j] + p[i] * v[j]);
+    const double x0 = t[k * RB + k + 1];
+    const double sigma = tree_sum(tmp, W);
+    if (sigma == 0.0) { e[k] = x0; continue; }
+    const double mu = sqrt(fma(x0, x0, sigma));
+    const double v0 = x0 <= 0.0 ? x0 - mu : x0 + mu;
+    e[k] = x0 <= 0.0 ? mu : -mu;
+    const double beta = 2.0 / fma(v0, v0, sigma);
+    for (int l = 0; l < W; ++l) v[l] = l == k + 1 ? v0 : xs[l];
+    for (int l = 0; l < W; ++l) {
+      double s4[4] = {0.0, 0.0, 0.0, 0.0};
+      if (l < RB)
+        for (int j = 0; j < RB; ++j) s4[j & 3] = fma(t[j * RB + l], v[j], s4[j & 3]);
+      p[l] = (l > k && l < n) ? beta * ((s4[0] + s4[1]) + (s4[2] + s4[3])) : 0.0;
+      tmp[l] = p[l] * v[l];
+    }
+    const double K = (0.5 * beta) * tree_sum(tmp, W);
+    for (int l = k + 1; l < n; ++l) p[l] = fma(-K, v[l], p[l]);          /* w = p - K v */
+    for (int l = k + 1; l < n; ++l)
+      for (int j = k + 1; j < n; ++j) t[j * RB + l] = t[j * RB + l] - (v[l] * p[j] + p[l] * v[j]);
   }
-  for (int i = 0; i < n; ++i) d[i] = a[i * n + i];
-  if (n >= 2) e[n - 2] = a[(n - 1) * n + (n - 2)];
-  free(v); free(p);
+  for (int i = 0; i < n; ++i) d[i] = t[i * RB + i];
+  if (n >= 2) e[n - 2] = t[(n - 2) * RB + (n - 1)];
+  free(t); free(v); free(p); free(xs); free(tmp);
 }
 
 /* number of eigenvalues of the tridiagonal (d, e2 = e^2) below x */
@@ -520,12 +544,9 @@ static double sturm_find(int n, const double *d, const double *e2, double pivmin
 int ptno_sym_extremes(int R, const double *V, double *L, double *mu)
 {
   const int n = R;
-  double *a = (double *)malloc((size_t)n * n * sizeof(double));
   double *d = (double *)malloc(sizeof(double) * n), *e = (double *)calloc((size_t)n, sizeof(double));
   double *e2 = (double *)calloc((size_t)n, sizeof(double));
-  for (int i = 0; i < n; ++i)
-    for (int j = 0; j < n; ++j) a[i * n + j] = V[i + (int64_t)n * j];
-  tridiagonalize(n, a, d, e);
+  tridiagonalize(n, V, d, e);
   double emax = 1.0;
   for (int i = 0; i + 1 < n; ++i) { e2[i] = e[i] * e[i]; if (e2[i] > emax) emax = e2[i]; }
   double pivmin = PTNO_PIVMIN_BASE * emax;
@@ -555,7 +576,7 @@ int ptno_sym_extremes(int R, const double *V, double *L, double *mu)
   double amax = fabs(lmin) > fabs(lmax) ? fabs(lmin) : fabs(lmax);
   *L = amax;
   *mu = amin;
-  free(a); free(d); free(e); free(e2);
+  free(d); free(e); free(e2);
   return targets;
 }
 
@@ -603,6 +624,7 @@ int ptno_nesterov_mnls(int64_t I, int R, const double *Z, const double *W, doubl
   if (L_out) *L_out = L;
   if (mu_out) *mu_out = mu;
   ptno_glambda(mu, &L, &lambda, &q);
+  const double invL = 1.0 / L;
 
   size_t n = (size_t)(I * R);
   double *Zt = (double *)malloc((size_t)R * R * sizeof(double));
@@ -634,7 +656,10 @@ int ptno_nesterov_mnls(int64_t I, int R, const double *Z, const double *W, doubl
       }
     if ((gmax <= d1 && gmin >= -d2) || iter >= max_inner) break;
     for (size_t e = 0; e < n; ++e) {
-      double t = Y[e] - G[e] / L;
+      /* G / L as RN(G / L): q0 = G * RN(1/L) and one correction with the exact residual
+         G - q0 L (Markstein's theorem), the GPU kernel's sequence */
+      double q0 = G[e] * invL;
+      double t = Y[e] - fma(fma(-q0, L, G[e]), invL, q0);
       An[e] = t > 0.0 ? t : 0.0;                                           /* cwiseMax(0) */
     }
     double na = ptno_update_alpha(alpha, q);

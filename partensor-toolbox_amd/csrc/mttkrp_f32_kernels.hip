@@ -44,8 +44,10 @@ __device__ __forceinline__ void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
 //   kperm 0:  k = 16 s16 + 8 (l/32) + j                       (lane pairs split 8 + 8)
 //   kperm 1:  k = 32 (s16/2) + 16 (l/32) + 8 (s16%2) + j       (lane pairs split 16 + 16)
 // W(k)[c] = prod_q fac_q[d_q(k)][c]; wf: FP32 row-major [k][32 NT] up to the padded K.
-__global__ void k_pack_w(KrArgs32 w, int64_t K, int64_t ksteps, int NT, int ldr, int kperm, bf16x8 *whi,
-                         bf16x8 *wlo, float *wf) {
+// Column tiles t >= tsplit hold a second operand w2 (column c - 32 tsplit of its factors):
+// one contraction then serves two sets of factors.
+__global__ void k_pack_w(KrArgs32 w1, KrArgs32 w2, int tsplit, int64_t K, int64_t ksteps, int NT, int ldr,
+                         int kperm, bf16x8 *whi, bf16x8 *wlo, float *wf) {
   const int64_t n = ksteps * NT * 64;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
     const int l = (int)(e % 64);
@@ -53,18 +55,21 @@ __global__ void k_pack_w(KrArgs32 w, int64_t K, int64_t ksteps, int NT, int ldr,
     const int t = (int)(st % NT);
     const int64_t s16 = st / NT;
     const int c = 32 * t + (l % 32);
+    const bool second = t >= tsplit;
+    const KrArgs32 &w = second ? w2 : w1;
+    const int cf = second ? c - 32 * tsplit : c;
     const int64_t kbase = kperm ? 32 * (s16 / 2) + 16 * (l / 32) + 8 * (s16 % 2) : 16 * s16 + 8 * (l / 32);
     bf16x8 h, lo;
     for (int j = 0; j < 8; ++j) {
       const int64_t k = kbase + j;
       double v = 0.0;
-      if (k < K && c < ldr) {
+      if (k < K && cf < ldr) {
         uint32_t rem = (uint32_t)k;
         v = 1.0;
         for (int q = 0; q < w.nm; ++q) {
           uint32_t d = rem % w.dim[q];
           rem /= w.dim[q];
-          v *= w.fac[q][(int64_t)d * ldr + c];
+          v *= w.fac[q][(int64_t)d * ldr + cf];
         }
       }
       float f = (float)v;
@@ -479,13 +484,16 @@ KrArgs32 to_args32(const KrSpec &w) {
 
 int64_t wpack_elems(int64_t K, int RPF) { return ((K + 31) / 32) * 2 * (RPF / 32) * 64; }
 
-void pack_w(hipStream_t s, const KrSpec &w, int64_t K, int RPF, int ldr, int kperm, void *whi, void *wlo, float *wf) {
+void pack_w(hipStream_t s, const KrSpec &w, const KrSpec *w2, int64_t K, int RPF, int ldr, int kperm, void *whi,
+            void *wlo, float *wf) {
   const int NT = RPF / 32;
+  const int tsplit = w2 ? NT / 2 : NT;
   const int64_t ksteps = ((K + 31) / 32) * 2;
   const int64_t n = ksteps * NT * 64;
   int64_t g = (n + 255) / 256;
   if (g > 8192) g = 8192;
-  k_pack_w<<<(unsigned)g, 256, 0, s>>>(to_args32(w), K, ksteps, NT, ldr, kperm, (bf16x8 *)whi, (bf16x8 *)wlo, wf);
+  k_pack_w<<<(unsigned)g, 256, 0, s>>>(to_args32(w), to_args32(w2 ? *w2 : w), tsplit, K, ksteps, NT, ldr, kperm,
+                                       (bf16x8 *)whi, (bf16x8 *)wlo, wf);
 }
 
 void gemm_cm_f32(hipStream_t s, GemmMode mode, const float *x32, int64_t M, int64_t K, int RPF, int splits,
@@ -494,13 +502,17 @@ void gemm_cm_f32(hipStream_t s, GemmMode mode, const float *x32, int64_t M, int6
   kchunk = (kchunk + 15) / 16 * 16;
   dim3 grid((unsigned)((M + 255) / 256), (unsigned)splits);
   const bf16x8 *h = (const bf16x8 *)whi, *l = (const bf16x8 *)wlo;
+#define PTN_CM32(MODEV, NTV) k_gemm_cm_f32<MODEV, NTV><<<grid, 256, 0, s>>>(x32, M, K, kchunk, h, l, wf, out)
   if (mode == kGemmBf16x3) {
-    if (RPF == 32) k_gemm_cm_f32<kGemmBf16x3, 1><<<grid, 256, 0, s>>>(x32, M, K, kchunk, h, l, nullptr, out);
-    else           k_gemm_cm_f32<kGemmBf16x3, 2><<<grid, 256, 0, s>>>(x32, M, K, kchunk, h, l, nullptr, out);
+    if (RPF == 32) PTN_CM32(kGemmBf16x3, 1);
+    else if (RPF == 64) PTN_CM32(kGemmBf16x3, 2);
+    else PTN_CM32(kGemmBf16x3, 4);
   } else {
-    if (RPF == 32) k_gemm_cm_f32<kGemmFp32, 1><<<grid, 256, 0, s>>>(x32, M, K, kchunk, nullptr, nullptr, wf, out);
-    else           k_gemm_cm_f32<kGemmFp32, 2><<<grid, 256, 0, s>>>(x32, M, K, kchunk, nullptr, nullptr, wf, out);
+    if (RPF == 32) PTN_CM32(kGemmFp32, 1);
+    else if (RPF == 64) PTN_CM32(kGemmFp32, 2);
+    else PTN_CM32(kGemmFp32, 4);
   }
+#undef PTN_CM32
 }
 
 void gemm_ck_f32(hipStream_t s, GemmMode mode, const float *x32, int64_t L, int64_t C, int RPF, const void *whi,

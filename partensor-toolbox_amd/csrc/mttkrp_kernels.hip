@@ -246,6 +246,76 @@ __global__ __launch_bounds__(256) void k_contract_rows(const TT *__restrict__ T,
   }
 }
 
+// ---------------------------------------------------------------------------
+// contract2: a node with two digits (rows m = i0 + D0 i1, ld values per row) contracted
+// over one digit against the other digit's factor rows (a leaf TTV of the dimension
+// tree, TensorPartialProduct_R, TensorOperations.hpp:196-213).  Thread (q, c): row
+// group q = tid / 32, column c (+ 32 t for the column tiles).  Loads are issued four
+// rows at a time; the order of the sums is fixed.
+//   outmode 0: block (8 consecutive i0, i1-split), out[i0][r] = sum_i1 Y[m][r] F[i1][r]
+//   outmode 1: block (one i1, i0-split), 8 row groups summed in LDS in order
+// ---------------------------------------------------------------------------
+template <typename TT>
+__global__ __launch_bounds__(256) void k_contract2(const TT *__restrict__ Y, int64_t ld, int64_t D0, int64_t D1,
+                                                   int outmode, const double *__restrict__ F, int ldr, int R,
+                                                   int64_t slen, double *__restrict__ part) {
+  __shared__ double red[8][32];
+  const int q = threadIdx.x >> 5, c0 = threadIdx.x & 31;
+  const int NT = (R + 31) / 32;
+  if (outmode == 0) {
+    const int64_t i0 = (int64_t)blockIdx.x * 8 + q;
+    const int64_t b = (int64_t)blockIdx.y * slen, e = b + slen < D1 ? b + slen : D1;
+    for (int t = 0; t < NT; ++t) {
+      const int c = 32 * t + c0;
+      if (i0 >= D0 || c >= ldr) continue;
+      double acc = 0.0;
+      int64_t i1 = b;
+      for (; i1 + 4 <= e; i1 += 4) {
+        const double y0 = (double)Y[(i0 + D0 * i1) * ld + c], y1 = (double)Y[(i0 + D0 * (i1 + 1)) * ld + c];
+        const double y2 = (double)Y[(i0 + D0 * (i1 + 2)) * ld + c], y3 = (double)Y[(i0 + D0 * (i1 + 3)) * ld + c];
+        const double f0 = F[i1 * ldr + c], f1 = F[(i1 + 1) * ldr + c];
+        const double f2 = F[(i1 + 2) * ldr + c], f3 = F[(i1 + 3) * ldr + c];
+        acc = fma(y0, f0, acc);
+        acc = fma(y1, f1, acc);
+        acc = fma(y2, f2, acc);
+        acc = fma(y3, f3, acc);
+      }
+      for (; i1 < e; ++i1) acc = fma((double)Y[(i0 + D0 * i1) * ld + c], F[i1 * ldr + c], acc);
+      part[((int64_t)blockIdx.y * D0 + i0) * ldr + c] = acc;
+    }
+    return;
+  }
+  const int64_t i1 = blockIdx.x;
+  const int64_t b = (int64_t)blockIdx.y * slen, e = b + slen < D0 ? b + slen : D0;
+  for (int t = 0; t < NT; ++t) {
+    const int c = 32 * t + c0;
+    double acc = 0.0;
+    if (c < ldr) {
+      const TT *row = Y + (D0 * i1) * ld + c;
+      int64_t i0 = b + q;
+      for (; i0 + 24 < e; i0 += 32) {
+        const double y0 = (double)row[i0 * ld], y1 = (double)row[(i0 + 8) * ld];
+        const double y2 = (double)row[(i0 + 16) * ld], y3 = (double)row[(i0 + 24) * ld];
+        const double f0 = F[i0 * ldr + c], f1 = F[(i0 + 8) * ldr + c];
+        const double f2 = F[(i0 + 16) * ldr + c], f3 = F[(i0 + 24) * ldr + c];
+        acc = fma(y0, f0, acc);
+        acc = fma(y1, f1, acc);
+        acc = fma(y2, f2, acc);
+        acc = fma(y3, f3, acc);
+      }
+      for (; i0 < e; i0 += 8) acc = fma((double)row[i0 * ld], F[i0 * ldr + c], acc);
+    }
+    __syncthreads();
+    red[q][c0] = acc;
+    __syncthreads();
+    if (q == 0 && c < ldr) {
+      double sum = red[0][c0];
+      for (int g = 1; g < 8; ++g) sum += red[g][c0];
+      part[((int64_t)blockIdx.y * D1 + i1) * ldr + c] = sum;
+    }
+  }
+}
+
 __global__ void k_reduce_splits(const double *__restrict__ partials, int splits, int64_t n, double *__restrict__ out) {
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
     double s = 0.0;
@@ -305,6 +375,15 @@ void contract_rows(hipStream_t s, const double *T64, const float *T32, int64_t l
   dim3 grid((unsigned)spec.dim[outmode], (unsigned)splits);
   if (T64) k_contract_rows<double><<<grid, 256, 0, s>>>(T64, ldt, sp, RP, ldr, cchunk, Cother, partials);
   else     k_contract_rows<float><<<grid, 256, 0, s>>>(T32, ldt, sp, RP, ldr, cchunk, Cother, partials);
+}
+
+void contract2(hipStream_t s, const double *Y64, const float *Y32, int64_t ld, int64_t D0, int64_t D1, int outmode,
+               const double *F, int ldr, int R, int splits, double *part) {
+  const int64_t D = outmode == 0 ? D1 : D0;             // contracted extent
+  const int64_t slen = (D + splits - 1) / splits;
+  dim3 grid(outmode == 0 ? (unsigned)((D0 + 7) / 8) : (unsigned)D1, (unsigned)splits);
+  if (Y64) k_contract2<double><<<grid, 256, 0, s>>>(Y64, ld, D0, D1, outmode, F, ldr, R, slen, part);
+  else     k_contract2<float><<<grid, 256, 0, s>>>(Y32, ld, D0, D1, outmode, F, ldr, R, slen, part);
 }
 
 void reduce_splits(hipStream_t s, const double *partials, int splits, int64_t n, double *out) {

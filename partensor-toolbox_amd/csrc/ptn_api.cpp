@@ -21,6 +21,8 @@ using namespace ptn;
 struct ptn_handle_s {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;                // eigen / momentum set-up of the next NNLS, beside the contractions
+  hipEvent_t ev[2] = {nullptr, nullptr};     // [0] main -> side, [1] side -> main
   std::string err;
   int N = 0;
   int64_t dims[kMaxOrder] = {0};
@@ -109,8 +111,9 @@ struct Work {
   int arith;
   bool f32;                                  // FP32 tensor stream (arith F32 / BF16X3 / F32_F64)
   bool mfma32;                               // contractions on FP32-output kernels (F32 / BF16X3)
-  DevBuf<double> fac, old, nrm, acc, grams, agr, V, M, Aold, colsum, part, T, YL, nscr, scal, hist, betas, gpart;
-  DevBuf<float> T32, YL32, part32, wf;
+  DevBuf<double> fac, old, nrm, acc, grams, agr, Vm, Vacc, M, Aold, colsum, part, T, YL, nscr, scal, hist, betas,
+      gpart;
+  DevBuf<float> T32, YL32, YL2, part32, wf;
   DevBuf<uint16_t> whi, wlo;
   DevBuf<NnlsParams> prm;
   DevBuf<uint64_t> mask;
@@ -120,6 +123,7 @@ struct Work {
     const float *p32 = nullptr;
     int64_t ld = 0;
   } left, right;                             // dimension-tree nodes cached between the modes of a sweep
+  bool left_ready = false;                   // left node already formed for the coming mode 0 (line-search pass)
 
   Work(ptn_handle hh, int R_, int arith_, bool need_tree)
       : h(hh), N(hh->N), R(R_), ldr(round_up16(R_)), rpf((R_ + 31) / 32 * 32), arith(arith_) {
@@ -136,10 +140,11 @@ struct Work {
     }
     fsz = o;
     fac.alloc(h, fsz); old.alloc(h, fsz); nrm.alloc(h, fsz); acc.alloc(h, fsz);
-    grams.alloc(h, (size_t)N * R * R); agr.alloc(h, (size_t)N * R * R); V.alloc(h, (size_t)R * R);
+    grams.alloc(h, (size_t)N * R * R); agr.alloc(h, (size_t)N * R * R);
+    Vm.alloc(h, (size_t)(N + 1) * R * R); Vacc.alloc(h, (size_t)R * R);
     M.alloc(h, maxI * ldr); Aold.alloc(h, maxI * ldr); colsum.alloc(h, kMaxRank);
-    prm.alloc(h, kMaxOrder); mask.alloc(h, 4);
-    nscr.alloc(h, 2 * kMaxOrder * kMaxRank); scal.alloc(h, 8); betas.alloc(h, kMaxInner + 1);
+    prm.alloc(h, kMaxOrder + 1); mask.alloc(h, nnls_mask_words(maxI));
+    nscr.alloc(h, 2 * kMaxOrder * kMaxRank); scal.alloc(h, 8); betas.alloc(h, (size_t)(N + 1) * (kMaxInner + 1));
     gpart.alloc(h, (size_t)((maxI + 31) / 32) * R * R);
     const int64_t C = total / dims[0];
     const int64_t yl = dims[0] * dims[1];
@@ -158,6 +163,16 @@ struct Work {
     }
     part.alloc(h, part_cap);
   }
+  // buffers of the paired left-node pass (dimension tree, FP32-output contractions)
+  void enable_pair() {
+    const int64_t yl = dims[0] * dims[1];
+    YL2.alloc(h, yl * 2 * rpf);
+    if (N == 4 && 64 * yl * 2 * rpf > part32_cap) {
+      part32.~DevBuf(); new (&part32) DevBuf<float>();
+      part32_cap = 64 * yl * 2 * rpf;
+      part32.alloc(h, part32_cap);
+    }
+  }
   double *F(int k) { return fac.p + off[k]; }
   double *Fb(double *base, int k) { return base + off[k]; }
   double *G(int k) { return grams.p + (int64_t)k * R * R; }
@@ -165,20 +180,22 @@ struct Work {
   const float *x32() const { return arith == PTN_ARITH_F64 ? nullptr : h->x32; }
   GemmMode gmode() const { return arith == PTN_ARITH_BF16X3 ? kGemmBf16x3 : kGemmFp32; }
 
-  // Khatri-Rao operand of an FP32 contraction, packed in fragment order
-  void pack(const KrSpec &k, int64_t K, int kperm) {
-    const int64_t frag = wpack_elems(K, rpf);
+  // Khatri-Rao operand of an FP32 contraction, packed in fragment order; with k2 the
+  // operand carries both (2 rpf columns: k in the lower half, k2 in the upper).
+  void pack(const KrSpec &k, int64_t K, int kperm, const KrSpec *k2 = nullptr) {
+    const int width = k2 ? 2 * rpf : rpf;
+    const int64_t frag = wpack_elems(K, width);
     if (frag > wcap) {
       HIPCHK(hipStreamSynchronize(h->stream));
       whi.~DevBuf(); new (&whi) DevBuf<uint16_t>();
       wlo.~DevBuf(); new (&wlo) DevBuf<uint16_t>();
       wf.~DevBuf(); new (&wf) DevBuf<float>();
       whi.alloc(h, frag * 8); wlo.alloc(h, frag * 8);
-      wf.alloc(h, (size_t)((K + 31) / 32 * 32) * rpf);
+      wf.alloc(h, (size_t)((K + 31) / 32 * 32) * width);
       wcap = frag;
     }
     const bool x3 = arith == PTN_ARITH_BF16X3;
-    pack_w(h->stream, k, K, rpf, ldr, kperm, x3 ? (void *)whi.p : nullptr, x3 ? (void *)wlo.p : nullptr,
+    pack_w(h->stream, k, k2, K, width, ldr, kperm, x3 ? (void *)whi.p : nullptr, x3 ? (void *)wlo.p : nullptr,
            x3 ? nullptr : wf.p);
   }
 };
@@ -259,12 +276,23 @@ void right_node_fused(Work &w, double *base, double *M) {
 
 // M[i][r] = sum over the node's other digits of node[row][r] * (their factor rows)
 void contract_node(Work &w, const Node &nd, const KrSpec &sp, int outmode, double *M) {
-  hipStream_t s = w.h->stream;
+  ptn_handle h = w.h;
+  hipStream_t s = h->stream;
   const int64_t In = sp.dim[outmode];
   int64_t Cother = 1;
   for (int j = 0; j < sp.nm; ++j)
     if (j != outmode) Cother *= sp.dim[j];
+  if (sp.nm == 2) {
+    // two digits: coalesced row-block kernel (one output digit, one contracted digit)
+    const int64_t D0 = sp.dim[0], D1 = sp.dim[1];
+    const int S = outmode == 0 ? pick_splits((D0 + 7) / 8, D1 / 16, 64) : pick_splits(D1, D0 / 64, 64);
+    REQUIRE(S * In * w.ldr <= w.part_cap, PTN_ERR_INVALID, "partial buffer too small");
+    contract2(s, nd.p64, nd.p32, nd.ld, D0, D1, outmode, sp.fac[1 - outmode], w.ldr, w.R, S, w.part);
+    reduce_splits(s, w.part, S, In * w.ldr, M);
+    return;
+  }
   const int S = pick_splits(In, Cother / 64, 64);
+  REQUIRE(S * In * w.ldr <= w.part_cap, PTN_ERR_INVALID, "partial buffer too small");
   contract_rows(s, nd.p64, nd.p32, nd.ld, sp, outmode, w.ldr, S, w.part);
   reduce_splits(s, w.part, S, In * w.ldr, M);
 }
@@ -311,7 +339,9 @@ void mttkrp_tree(Work &w, double *base, int mode, double *M) {
   auto keep = [](Work::NodeRef &dst, const Node &n) { dst.p64 = n.p64; dst.p32 = n.p32; dst.ld = n.ld; };
   auto node = [](const Work::NodeRef &r) { return Node{r.p64, r.p32, r.ld}; };
   if (N == 3) {
-    if (mode == 0) keep(w.left, node_cm(w, kr_of(w, base, {2}), d[0] * d[1], d[2], 1, w.YL.p, w.YL32.p));
+    if (mode == 0 && !w.left_ready)
+      keep(w.left, node_cm(w, kr_of(w, base, {2}), d[0] * d[1], d[2], 1, w.YL.p, w.YL32.p));
+    w.left_ready = false;
     if (mode <= 1) {
       contract_node(w, node(w.left), kr_of(w, base, {0, 1}), mode, M);
       return;
@@ -324,13 +354,41 @@ void mttkrp_tree(Work &w, double *base, int mode, double *M) {
     contract_node(w, node(w.right), kr_of(w, base, {1, 2}), 1, M);
     return;
   }
-  if (mode == 0) {
+  if (mode == 0 && !w.left_ready) {
     const int S = pick_splits((d[0] * d[1] + 255) / 256, d[2] * d[3] / 256, 64);
     keep(w.left, node_cm(w, kr_of(w, base, {2, 3}), d[0] * d[1], d[2] * d[3], S, w.YL.p, w.YL32.p));
   }
+  w.left_ready = false;
   if (mode == 2) keep(w.right, node_ck(w, kr_of(w, base, {0, 1}), d[0] * d[1], d[2] * d[3]));
   if (mode <= 1) contract_node(w, node(w.left), kr_of(w, base, {0, 1}), mode, M);
   else contract_node(w, node(w.right), kr_of(w, base, {2, 3}), mode - 2, M);
+}
+
+// Line-search pass fused with the next sweep's left node: one pass over X forms
+// YL2[m] = [X x_rest A_rest | X x_rest Ahat_rest] (rest = modes 2.. of the current and of
+// the accelerated factors, 2 rpf columns per row), and Mhat (I_0 x ldr) =
+// sum_{i1} YL2hat[i0 + I0 i1] Ahat_1[i1] from the upper half, so <Mhat, Ahat_0> is the
+// fit term <X, [[Ahat]]> of accel_cost_function (CpdDimTree.hpp:174-181) summed in
+// another order.
+void left_node_pair(Work &w, double *cur, double *acc, double *Mhat) {
+  ptn_handle h = w.h;
+  hipStream_t s = h->stream;
+  const int64_t *d = w.dims;
+  const bool n3 = w.N == 3;
+  const KrSpec k1 = n3 ? kr_of(w, cur, {2}) : kr_of(w, cur, {2, 3});
+  const KrSpec k2 = n3 ? kr_of(w, acc, {2}) : kr_of(w, acc, {2, 3});
+  const int64_t Mrows = d[0] * d[1], K = w.total / Mrows;
+  const int width = 2 * w.rpf;
+  w.pack(k1, K, 0, &k2);
+  const int S = n3 ? 1 : pick_splits((Mrows + 255) / 256, K / 256, 64);
+  if (S == 1) {
+    gemm_cm_f32(s, w.gmode(), w.x32(), Mrows, K, width, 1, w.whi, w.wlo, w.wf, w.YL2);
+  } else {
+    REQUIRE(S * Mrows * width <= w.part32_cap, PTN_ERR_INVALID, "partial buffer too small");
+    gemm_cm_f32(s, w.gmode(), w.x32(), Mrows, K, width, S, w.whi, w.wlo, w.wf, w.part32);
+    reduce_splits_f32(s, w.part32, S, Mrows * width, w.YL2);
+  }
+  contract_node(w, Node{nullptr, w.YL2.p + w.rpf, width}, kr_of(w, acc, {0, 1}), 0, Mhat);
 }
 
 // Hadamard order of V_n: ascending (PartialCwiseProd, right-nested) for the standard
@@ -352,12 +410,29 @@ int v_order(int N, int mode, bool tree, int *list) {
   return n;
 }
 
-void compute_V(Work &w, double *gbase, int mode, bool tree) {
+// Inputs of one NNLS update: V_n (PartialCwiseProd of the Grams), its extreme
+// eigenvalues with the GLambda shift, and the momentum coefficients.  Slot n serves
+// mode n; slot N holds the mode-0 set-up for the accelerated factors while the line
+// search is undecided.
+struct ModeIn {
+  double *V;
+  NnlsParams *p;
+  double *betas;
+};
+
+ModeIn slot(Work &w, int s) {
+  return ModeIn{w.Vm.p + (int64_t)s * w.R * w.R, w.prm.p + s, w.betas.p + (int64_t)s * (kMaxInner + 1)};
+}
+
+void prep_mode(Work &w, const double *gbase, int mode, bool tree, const ModeIn &in, int max_inner,
+               hipStream_t s) {
   int list[kMaxOrder];
   int n = v_order(w.N, mode, tree, list);
   const double *gp[kMaxOrder];
   for (int j = 0; j < n; ++j) gp[j] = gbase + (int64_t)list[j] * w.R * w.R;
-  hadamard(w.h->stream, w.R, n, gp, w.V);
+  hadamard(s, w.R, n, gp, in.V);
+  eig_extremes(s, w.R, in.V, in.p);
+  nnls_prep(s, in.p, max_inner, in.betas);
 }
 
 void gram_of(Work &w, const double *A, int64_t I, double *G) { gram(w.h->stream, I, w.R, w.ldr, A, G, w.gpart); }
@@ -415,13 +490,14 @@ double cost_explicit(Work &w, const double *base) {
   return std::sqrt(read_scalar(h, w.scal.p + 3));
 }
 
-// update_factor, Cpd.hpp:189-228, nonnegativity branch on the device
-void update_nonneg(Work &w, int mode, const ptn_options &o) {
+// update_factor, Cpd.hpp:189-228, nonnegativity branch on the device (V_n, its
+// eigenvalues and the momentum coefficients prepared in `in`)
+void update_nonneg(Work &w, int mode, const ptn_options &o, const ModeIn &in) {
   ptn_handle h = w.h;
   hipStream_t s = h->stream;
   const int64_t I = w.dims[mode];
   double *A = w.F(mode);
-  NnlsParams *p = w.prm.p + mode;
+  NnlsParams *p = in.p;
   const int64_t need = (int64_t)(o.max_inner + 1) * w.maxI * w.ldr;
   if (need > w.hist_cap) {
     HIPCHK(hipStreamSynchronize(s));
@@ -431,10 +507,8 @@ void update_nonneg(Work &w, int mode, const ptn_options &o) {
     w.hist_cap = need;
   }
   HIPCHK(hipMemcpyAsync(w.Aold, A, I * w.ldr * sizeof(double), hipMemcpyDeviceToDevice, s));
-  eig_extremes(s, w.R, w.V, p);
-  nnls_prep(s, p, o.max_inner, w.betas, w.mask, w.colsum, w.R);
-  nnls_iterate(s, I, w.R, w.ldr, w.V, p, w.betas, w.M, w.Aold, o.nesterov_delta_1, o.nesterov_delta_2, o.max_inner,
-               w.mask, w.hist);
+  nnls_iterate(s, I, w.R, w.ldr, in.V, p, in.betas, w.M, w.Aold, o.nesterov_delta_1, o.nesterov_delta_2,
+               o.max_inner, w.mask, w.hist, w.colsum);
   nnls_select(s, I, w.R, w.ldr, w.hist, w.mask, o.max_inner, A, w.colsum, p);
   zero_col_fallback(s, I, w.R, w.ldr, w.colsum, w.Aold, A, p);
 }
@@ -475,6 +549,8 @@ ptn_status ptn_create(ptn_handle *out, int device) {
     REQUIRE(device >= 0 && device < n, PTN_ERR_INVALID, "no such device");
     HIPCHK(hipSetDevice(device));
     HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking));
+    for (hipEvent_t &e : h->ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HIPCHK(hipMalloc((void **)&h->scal, 16 * sizeof(double)));
     HIPCHK(hipMalloc((void **)&h->part, kNormParts * sizeof(double)));
   });
@@ -490,11 +566,15 @@ void ptn_destroy(ptn_handle h) {
   if (!h) return;
   (void)hipSetDevice(h->device);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->side) (void)hipStreamSynchronize(h->side);
   if (h->x64) (void)hipFree(h->x64);
   if (h->x32) (void)hipFree(h->x32);
   if (h->scal) (void)hipFree(h->scal);
   if (h->part) (void)hipFree(h->part);
   if (h->stream) (void)hipStreamDestroy(h->stream);
+  if (h->side) (void)hipStreamDestroy(h->side);
+  for (hipEvent_t e : h->ev)
+    if (e) (void)hipEventDestroy(e);
   delete h;
 }
 
@@ -637,8 +717,12 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
     REQUIRE(!tree || h->N == 3 || h->N == 4, PTN_ERR_UNSUPPORTED, "dimension tree built for N = 3, 4");
     for (int k = 0; k < h->N; ++k)
       REQUIRE(o.constraints[k] == PTN_NONNEGATIVITY, PTN_ERR_UNSUPPORTED, "only the nonnegativity constraint is built");
+    const bool pair = o.fuse_accel != 0;
+    REQUIRE(!pair || (tree && (o.arith == PTN_ARITH_F32 || o.arith == PTN_ARITH_BF16X3)), PTN_ERR_UNSUPPORTED,
+            "fuse_accel needs the dimension tree with F32 or BF16X3 contractions");
 
     Work w(h, R, o.arith, tree);
+    if (pair && o.acceleration) w.enable_pair();
     const int N = w.N, last = N - 1;
     upload_factors(w, init_packed, w.fac);
     for (int k = 0; k < N; ++k) gram_of(w, w.F(k), w.dims[k], w.G(k));
@@ -660,35 +744,57 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
     HIPCHK(hipMemcpyAsync(w.scal.p, &r.frob_sq, sizeof(double), hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
 
+    // Two streams: the side stream forms V_n, its eigenvalues and the momentum
+    // coefficients of mode n while the main stream contracts the tensor for mode n.
+    hipStream_t ms = h->stream, ss = h->side;
+    auto fork = [&] {      // side waits for the main stream's work so far
+      HIPCHK(hipEventRecord(h->ev[0], ms));
+      HIPCHK(hipStreamWaitEvent(ss, h->ev[0], 0));
+    };
+    auto join = [&] {      // main waits for the side stream's work so far
+      HIPCHK(hipEventRecord(h->ev[1], ss));
+      HIPCHK(hipStreamWaitEvent(ms, h->ev[1], 0));
+    };
+    ModeIn sel[kMaxOrder];
+    int sel0 = 0;                    // slot of mode 0's set-up, 0 or N (prepared during the line search)
+    bool prepped0 = false;
     auto t0 = std::chrono::steady_clock::now();
-    NnlsParams hp[kMaxOrder];
+    NnlsParams hp[kMaxOrder + 1];
     for (;;) {
       r.ao_iter++;
       ptn_sweep_log lg{};
       for (int n = 0; n < N; ++n) {
-        compute_V(w, w.grams, n, tree);
+        if (!(n == 0 && prepped0)) {
+          sel[n] = slot(w, n);
+          if (n == 0) sel0 = 0;
+          fork();
+          prep_mode(w, w.grams, n, tree, sel[n], o.max_inner, ss);
+        }
         if (tree) mttkrp_tree(w, w.fac, n, w.M);
         else mttkrp_standard(w, w.fac, n, w.M);
-        update_nonneg(w, n, o);
+        join();
+        update_nonneg(w, n, o, sel[n]);
         gram_of(w, w.F(n), w.dims[n], w.G(n));
         if (n == last)
-          cost_gram(h->stream, w.dims[last] * w.ldr, w.M, w.F(last), R, w.V, w.G(last), w.scal.p, w.scal.p + 1, 0);
+          cost_gram(ms, w.dims[last] * w.ldr, w.M, w.F(last), R, sel[last].V, w.G(last), w.scal.p, w.scal.p + 1, 0);
       }
+      prepped0 = false;
       if (o.normalization && !all_orth) {
         double *gp[kMaxOrder], *fp[kMaxOrder];
         for (int k = 0; k < N; ++k) { gp[k] = w.G(k); fp[k] = w.F(k); }
         normalize(h->stream, N, w.dims, R, w.ldr, wf, gp, fp, w.nscr);
       }
-      HIPCHK(hipMemcpyAsync(hp, w.prm.p, sizeof(NnlsParams) * N, hipMemcpyDeviceToHost, h->stream));
+      HIPCHK(hipMemcpyAsync(hp, w.prm.p, sizeof(NnlsParams) * (N + 1), hipMemcpyDeviceToHost, h->stream));
       r.f_value = read_scalar(h, w.scal.p + 1);
       r.rel_cost = r.f_value / std::sqrt(r.frob_sq);
       lg.f_value = r.f_value;
       lg.rel_cost = r.rel_cost;
       for (int n = 0; n < N; ++n) {
-        lg.nnls_iters[n] = hp[n].iters;
-        lg.zero_col_fallback[n] = hp[n].fallback;
-        lg.lipschitz[n] = hp[n].L0;
-        lg.mu[n] = hp[n].mu0;
+        const NnlsParams &q = hp[n == 0 ? sel0 : n];
+        lg.nnls_iters[n] = q.iters;
+        lg.zero_col_fallback[n] = q.fallback;
+        lg.lipschitz[n] = q.L0;
+        lg.mu[n] = q.mu0;
       }
       const bool stop = r.rel_cost < o.threshold_error || r.ao_iter >= o.max_iter;
       if (!stop && o.acceleration) {
@@ -698,14 +804,26 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
           double step = std::pow((double)(r.ao_iter + 1), 1.0 / (double)r.accel_coeff);
           axpy_accel(h->stream, w.fsz, step, w.old, w.fac, w.acc);
           for (int k = 0; k < N; ++k) gram_of(w, w.Fb(w.acc, k), w.dims[k], w.agr.p + (int64_t)k * R * R);
-          mttkrp_standard(w, w.acc, last, w.M);
+          // accel_cost_function (CpdDimTree.hpp:174-181): its fit term from the last mode's
+          // MTTKRP, or from mode 0 of the paired left-node pass
+          const int cm = pair ? 0 : last;
+          if (pair) {
+            // mode 0 of the next sweep for either outcome, beside the paired pass
+            fork();
+            prep_mode(w, w.grams, 0, tree, slot(w, 0), o.max_inner, ss);
+            prep_mode(w, w.agr, 0, tree, slot(w, N), o.max_inner, ss);
+            left_node_pair(w, w.fac, w.acc, w.M);
+          } else {
+            mttkrp_standard(w, w.acc, last, w.M);
+          }
           int list[kMaxOrder];
-          int nl = v_order(N, last, false, list);
+          int nl = v_order(N, cm, false, list);
           const double *gp[kMaxOrder];
           for (int j = 0; j < nl; ++j) gp[j] = w.agr.p + (int64_t)list[j] * R * R;
-          hadamard(h->stream, R, nl, gp, w.V);
-          cost_gram(h->stream, w.dims[last] * w.ldr, w.M, w.Fb(w.acc, last), R, w.V, w.agr.p + (int64_t)last * R * R,
+          hadamard(ms, R, nl, gp, w.Vacc);
+          cost_gram(ms, w.dims[cm] * w.ldr, w.M, w.Fb(w.acc, cm), R, w.Vacc, w.agr.p + (int64_t)cm * R * R,
                     w.scal.p, w.scal.p + 2, 1);
+          if (pair) join();          // the side stream has read the Grams before they change
           double fa = read_scalar(h, w.scal.p + 2);
           lg.accel_tried = 1;
           lg.f_accel = fa;
@@ -718,6 +836,16 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
           } else {
             r.accel_fail++;
           }
+          if (pair) {
+            // the next sweep's left node and mode-0 set-up: those that match the kept factors
+            w.left.p64 = nullptr;
+            w.left.p32 = w.YL2.p + (lg.accel_accepted ? w.rpf : 0);
+            w.left.ld = 2 * w.rpf;
+            w.left_ready = true;
+            sel0 = lg.accel_accepted ? N : 0;
+            sel[0] = slot(w, sel0);
+            prepped0 = true;
+          }
           if (r.accel_fail == 5) {
             r.accel_fail = 0;
             r.accel_coeff++;
@@ -725,7 +853,7 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
         }
         HIPCHK(hipMemcpyAsync(w.old, w.nrm, w.fsz * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
       }
-      HIPCHK(hipStreamSynchronize(h->stream));
+      HIPCHK(hipStreamSynchronize(ms));
       lg.t_end = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
       if (log && r.ao_iter - 1 < log_cap) log[r.ao_iter - 1] = lg;
       if (stop) break;
@@ -805,7 +933,7 @@ ptn_status ptn_nesterov_mnls(ptn_handle h, int64_t I, int R, const double *Z, co
     DevBuf<double> hist(h, (size_t)(max_inner + 1) * I * ldr), betas(h, kMaxInner + 1);
     DevBuf<double> colsum(h, kMaxRank);
     DevBuf<NnlsParams> p(h, 1);
-    DevBuf<uint64_t> mask(h, 4);
+    DevBuf<uint64_t> mask(h, nnls_mask_words(I));
     std::vector<double> zr((size_t)R * R);
     for (int a = 0; a < R; ++a)
       for (int b = 0; b < R; ++b) zr[a * R + b] = Z[a + (int64_t)R * b];
@@ -815,8 +943,9 @@ ptn_status ptn_nesterov_mnls(ptn_handle h, int64_t I, int R, const double *Z, co
     HIPCHK(hipMemcpyAsync(tmp.p, A, I * R * sizeof(double), hipMemcpyHostToDevice, h->stream));
     colmajor_to_rows(h->stream, I, R, ldr, tmp.p, dA.p);
     eig_extremes(h->stream, R, dZ.p, p.p);
-    nnls_prep(h->stream, p.p, max_inner, betas.p, mask.p, colsum.p, R);
-    nnls_iterate(h->stream, I, R, ldr, dZ.p, p.p, betas.p, dW.p, dA.p, delta_1, delta_2, max_inner, mask.p, hist.p);
+    nnls_prep(h->stream, p.p, max_inner, betas.p);
+    nnls_iterate(h->stream, I, R, ldr, dZ.p, p.p, betas.p, dW.p, dA.p, delta_1, delta_2, max_inner, mask.p, hist.p,
+                 colsum.p);
     nnls_select(h->stream, I, R, ldr, hist.p, mask.p, max_inner, dB.p, colsum.p, p.p);
     rows_to_colmajor(h->stream, I, R, ldr, dB.p, tmp.p);
     HIPCHK(hipMemcpyAsync(A, tmp.p, I * R * sizeof(double), hipMemcpyDeviceToHost, h->stream));

@@ -65,6 +65,10 @@ void gemm_ck_f64(hipStream_t s, const double *x64, const float *x32, int64_t L, 
 void contract_rows(hipStream_t s, const double *T64, const float *T32, int64_t ldt, const KrSpec &spec,
                    int outmode, int ldr, int splits, double *partials);
 void reduce_splits(hipStream_t s, const double *partials, int splits, int64_t n, double *out);
+// Two-digit node Y (rows i0 + D0 i1, ld values per row) contracted against the other
+// digit's factor F: part[split][i][ldr] for the kept digit i (outmode 0: i0, 1: i1).
+void contract2(hipStream_t s, const double *Y64, const float *Y32, int64_t ld, int64_t D0, int64_t D1, int outmode,
+               const double *F, int ldr, int R, int splits, double *part);
 
 // FP32-stream MFMA variants (bf16x3 split products or FP32 MFMA), FP32 outputs with
 // RPF = R rounded up to 32 columns.  The Khatri-Rao operand is packed once per
@@ -72,7 +76,9 @@ void reduce_splits(hipStream_t s, const double *partials, int splits, int64_t n,
 enum GemmMode { kGemmFp32 = 0, kGemmBf16x3 = 1 };
 int64_t wpack_elems(int64_t K, int RPF);   // number of 16-byte fragments (per hi / lo array)
 // kperm 0: fragments for gemm_cm_f32 (16-wide k steps); kperm 1: for gemm_ck_f32 (32-wide)
-void pack_w(hipStream_t s, const KrSpec &w, int64_t K, int RPF, int ldr, int kperm, void *whi, void *wlo, float *wf);
+// w2 (optional): a second operand packed into the upper half of the RPF columns.
+void pack_w(hipStream_t s, const KrSpec &w, const KrSpec *w2, int64_t K, int RPF, int ldr, int kperm, void *whi,
+            void *wlo, float *wf);
 void gemm_cm_f32(hipStream_t s, GemmMode mode, const float *x32, int64_t M, int64_t K, int RPF, int splits,
                  const void *whi, const void *wlo, const float *wf, float *out);
 // fuse_I1 > 0: C = I1 * I2 fibers, epilogue TTV with A1 (FP64 rows, ldr) into part[I2][ceil(I1/64)][ldr]
@@ -88,13 +94,14 @@ void hadamard(hipStream_t s, int R, int ng, const double *const *grams, double *
 // partials: ceil(I/32) x R x R doubles
 void gram(hipStream_t s, int64_t I, int R, int ldr, const double *A, double *G, double *partials);
 void eig_extremes(hipStream_t s, int R, const double *V, NnlsParams *p);
-// betas: max_inner doubles; resets mask and colsum
-void nnls_prep(hipStream_t s, const NnlsParams *p, int max_inner, double *betas, uint64_t *mask, double *colsum,
-               int R);
-// hist: (max_inner + 1) x I x ldr doubles
+// betas: max_inner doubles
+void nnls_prep(hipStream_t s, const NnlsParams *p, int max_inner, double *betas);
+// satisfaction masks of nnls_iterate: 4 words per block of 4 rows
+inline int64_t nnls_mask_words(int64_t I) { return (I + 3) / 4 * 4; }
+// hist: (max_inner + 1) x I x ldr doubles; mask: nnls_mask_words(I) words
 void nnls_iterate(hipStream_t s, int64_t I, int R, int ldr, const double *V, const NnlsParams *p,
                   const double *betas, const double *M, const double *A, double d1, double d2, int max_inner,
-                  uint64_t *mask, double *hist);
+                  uint64_t *mask, double *hist, double *colsum);   // zeroes colsum for nnls_select
 void nnls_select(hipStream_t s, int64_t I, int R, int ldr, const double *hist, const uint64_t *mask, int max_inner,
                  double *Anew, double *colsum, NnlsParams *p);
 void zero_col_fallback(hipStream_t s, int64_t I, int R, int ldr, const double *colsum, const double *Aold,

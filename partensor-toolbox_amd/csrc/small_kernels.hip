@@ -117,18 +117,21 @@ __device__ int sturm_count(int n, const double *d, const double *e2, double pivm
   return c;
 }
 
-// Multisection for the j-th smallest eigenvalue, run by one 32-lane half of wave 0:
+// Multisection for the j-th smallest eigenvalue, run by one 32-lane half of the wave:
 // lane m < 31 probes lo + (m+1)(hi-lo)/32 each step; the new bracket is the first
-// probe interval whose count exceeds j (the oracle scans the probes in order).
+// probe interval whose count exceeds j (the oracle scans the probes in order).  Lane
+// 31, idle otherwise, counts the eigenvalues below 0 in the first step (*k0).
 __device__ double sturm_find_half(int n, const double *d, const double *e2, double pivmin, double lo, double hi,
-                                  int j, int m, int half) {
+                                  int j, int m, int half, int *k0) {
   for (int it = 0; it < 64; ++it) {
     const double w = hi - lo;
     const double sc = fabs(lo) > fabs(hi) ? fabs(lo) : fabs(hi);
     const bool stop = w <= 2.0 * kEps * sc || w <= pivmin;
-    bool hit = false;
-    if (!stop && m < 31) hit = sturm_count(n, d, e2, pivmin, fma(w, (double)(m + 1) * 0.03125, lo)) > j;
-    const unsigned long long bal = __ballot(hit);
+    const bool probe = m < 31 ? !stop : (k0 != nullptr && it == 0);
+    const double x = m < 31 ? fma(w, (double)(m + 1) * 0.03125, lo) : 0.0;
+    const int cnt = probe ? sturm_count(n, d, e2, pivmin, x) : 0;
+    if (k0 && it == 0 && m == 31) *k0 = cnt;
+    const unsigned long long bal = __ballot(m < 31 && probe && cnt > j);
     const unsigned bits = (unsigned)(bal >> (32 * half)) & 0x7fffffffu;
     if (stop) break;
     const int ms = bits ? __builtin_ctz(bits) : 31;
@@ -141,114 +144,122 @@ __device__ double sturm_find_half(int n, const double *d, const double *e2, doub
   return 0.5 * (lo + hi);
 }
 
+// Cross-lane move of a double through DPP (both halves; lanes without a source read 0,
+// lanes outside ROWMASK read 0).
+template <int CTRL, int ROWMASK = 0xf>
+__device__ __forceinline__ double dpp_d(double x) {
+  int2 v = __builtin_bit_cast(int2, x);
+  v.x = __builtin_amdgcn_update_dpp(0, v.x, CTRL, ROWMASK, 0xf, true);
+  v.y = __builtin_amdgcn_update_dpp(0, v.y, CTRL, ROWMASK, 0xf, true);
+  return __builtin_bit_cast(double, v);
+}
+
+// Sum over the 64 lanes as a balanced tree over consecutive lanes,
+// ((x0 + x1) + (x2 + x3)) + ...: quad swaps, row shifts by 4 and 8, then the row
+// broadcasts; every addition has the operands of the tree's node (in either order,
+// which IEEE addition does not distinguish).  The total lands in lane 63 and is
+// returned to every lane.
+__device__ __forceinline__ double tree64(double x) {
+  x = x + dpp_d<0xB1>(x);          // quad_perm [1,0,3,2]
+  x = x + dpp_d<0x4E>(x);          // quad_perm [2,3,0,1]
+  x = x + dpp_d<0x114>(x);         // row_shr:4  (lane 4q+3 of each 8 sums its 8)
+  x = x + dpp_d<0x118>(x);         // row_shr:8  (lane 15 of each row sums the row)
+  x = x + dpp_d<0x142, 0xA>(x);    // row_bcast:15 into rows 1, 3
+  x = x + dpp_d<0x143, 0xC>(x);    // row_bcast:31 into rows 2, 3
+  return bcast_lane(x, 63);
+}
+
 // Extreme singular values (= max / min |eigenvalue|) of the symmetric V: Householder
-// tridiagonalization (Golub & Van Loan Alg. 8.3.1) in LDS, then Sturm multisection
-// (Alg. 8.4.1) on wave 0, the two halves bracketing two eigenvalues at once.  Sums
-// run in index order with fma, matching the CPU restatement operation by operation;
-// loops are unrolled to RB with zero padding (an added fma(x, 0, s) leaves s unchanged,
-// s never being -0 here) so their LDS loads issue together.
+// tridiagonalization (Golub & Van Loan Alg. 8.3.1) by one wave with the matrix in LDS,
+// then Sturm multisection (Alg. 8.4.1), the two 32-lane halves bracketing two
+// eigenvalues at once.  Lane l owns index l.  The working matrix t holds V as loaded
+// (t[r][c] = V[r][c]); step k reads row k as the column to eliminate and forms
+// p_l = sum_j t[j][l] v_j, so every LDS access is lane-contiguous or a broadcast.
+// Sums: 64-lane trees (tree64) for sigma and p.v, four interleaved chains over the RB
+// padded rows for the matrix-vector product; the CPU restatement performs the same
+// operations in the same order, padding included.
 template <int RB>
-__global__ __launch_bounds__(256) void k_eig_extremes(int R, const double *__restrict__ V, NnlsParams *prm) {
-  __shared__ double a[RB * RB];
-  __shared__ double v[RB], p[RB], d[RB], e[RB], e2[RB];
-  __shared__ double sh_beta, sh_K, sh_piv, sh_glo, sh_ghi, sh_res[4];
-  __shared__ int sh_skip, sh_k0;
-  const int n = R, tid = threadIdx.x;
-  for (int x = tid; x < RB * RB; x += blockDim.x) {
-    const int i = x / RB, j = x % RB;
-    a[x] = (i < n && j < n) ? V[i * n + j] : 0.0;
+__global__ __launch_bounds__(64) void k_eig_extremes(int R, const double *__restrict__ V, NnlsParams *prm) {
+  __shared__ double t[RB * RB];
+  __shared__ double vb[RB], pb[RB], d[RB], e[RB], e2[RB];
+  __shared__ double sh_res[4];
+  __shared__ int sh_k0;
+  const int n = R, l = threadIdx.x;
+  for (int x = l; x < RB * RB; x += 64) {
+    const int r = x / RB, c = x % RB;
+    t[x] = (r < n && c < n) ? V[r * n + c] : 0.0;
   }
-  if (tid < RB) { e[tid] = 0.0; v[tid] = 0.0; p[tid] = 0.0; }
+  if (l < RB) { e[l] = 0.0; vb[l] = 0.0; pb[l] = 0.0; }
   __syncthreads();
+  const bool own = l < RB;
   for (int k = 0; k + 2 < n; ++k) {
-    if (tid == 0) {
-      double sigma = 0.0;
-#pragma unroll
-      for (int i = 0; i < RB; ++i) {
-        const double x = (i >= k + 2 && i < n) ? a[i * RB + k] : 0.0;
-        sigma = fma(x, x, sigma);
-      }
-      const double x0 = a[(k + 1) * RB + k];
-      if (sigma == 0.0) {
-        e[k] = x0;
-        sh_skip = 1;
-      } else {
-        const double mu = sqrt(fma(x0, x0, sigma));
-        const double v0 = x0 <= 0.0 ? x0 - mu : x0 + mu;
-        e[k] = x0 <= 0.0 ? mu : -mu;
-        sh_beta = 2.0 / fma(v0, v0, sigma);
-        sh_skip = 0;
-        v[k + 1] = v0;
-      }
-    } else if (tid >= k + 2 && tid < n) {
-      v[tid] = a[tid * RB + k];
+    const double xi = (l >= k + 2 && l < n) ? t[k * RB + l] : 0.0;
+    const double x0 = t[k * RB + k + 1];
+    const double sigma = tree64(xi * xi);
+    if (sigma == 0.0) {                                   // wave-uniform
+      if (l == 0) e[k] = x0;
+      continue;
     }
-    if (tid == k) v[k] = 0.0;                 // entries <= k stay zero from here on
+    const double mu = sqrt(fma(x0, x0, sigma));
+    const double v0 = x0 <= 0.0 ? x0 - mu : x0 + mu;
+    if (l == 0) e[k] = x0 <= 0.0 ? mu : -mu;
+    const double beta = 2.0 / fma(v0, v0, sigma);
+    const double vi = l == k + 1 ? v0 : xi;
+    if (own) vb[l] = vi;
     __syncthreads();
-    if (sh_skip) continue;                    // block-uniform
-    const double beta = sh_beta;
-    if (tid > k && tid < n) {
-      double sacc = 0.0;
+    double s4[4] = {0.0, 0.0, 0.0, 0.0};
+    if (own) {
 #pragma unroll
-      for (int j = 0; j < RB; ++j) sacc = fma(a[tid * RB + j], v[j], sacc);
-      p[tid] = beta * sacc;
+      for (int j = 0; j < RB; ++j) s4[j & 3] = fma(t[j * RB + l], vb[j], s4[j & 3]);
     }
+    const bool act = l > k && l < n;
+    double pi = act ? beta * ((s4[0] + s4[1]) + (s4[2] + s4[3])) : 0.0;
+    const double K = (0.5 * beta) * tree64(pi * vi);
+    if (act) pi = fma(-K, vi, pi);
+    if (own) pb[l] = pi;
     __syncthreads();
-    if (tid == 0) {
-      double t = 0.0;
-#pragma unroll
-      for (int i = 0; i < RB; ++i) t = fma(p[i], v[i], t);
-      sh_K = (0.5 * beta) * t;
-    }
-    __syncthreads();
-    if (tid > k && tid < n) p[tid] = fma(-sh_K, v[tid], p[tid]);
-    __syncthreads();
-    const int m = n - k - 1;
-    for (int x = tid; x < m * m; x += blockDim.x) {
-      const int i = k + 1 + x / m, j = k + 1 + x % m;
-      a[i * RB + j] = a[i * RB + j] - (v[i] * p[j] + p[i] * v[j]);
+    if (act) {
+#pragma unroll 8
+      for (int j = k + 1; j < n; ++j) t[j * RB + l] = t[j * RB + l] - (vi * pb[j] + pi * vb[j]);
     }
     __syncthreads();
   }
-  if (tid < n) d[tid] = a[tid * RB + tid];
-  if (tid == 0 && n >= 2) e[n - 2] = a[(n - 1) * RB + (n - 2)];
+  if (l < n) d[l] = t[l * RB + l];
+  if (l == 0 && n >= 2) e[n - 2] = t[(n - 2) * RB + (n - 1)];
   __syncthreads();
-  if (tid == 0) {
-    double emax = 1.0;
-    for (int i = 0; i + 1 < n; ++i) {
-      e2[i] = e[i] * e[i];
-      if (e2[i] > emax) emax = e2[i];
-    }
-    const double pivmin = kPivminBase * emax;
-    double glo = d[0], ghi = d[0];
-    for (int i = 0; i < n; ++i) {
-      const double r = (i > 0 ? fabs(e[i - 1]) : 0.0) + (i + 1 < n ? fabs(e[i]) : 0.0);
-      const double lo = d[i] - r, hi = d[i] + r;
-      if (lo < glo) glo = lo;
-      if (hi > ghi) ghi = hi;
-    }
-    const double pad = 2.0 * kEps * (fabs(glo) > fabs(ghi) ? fabs(glo) : fabs(ghi)) + pivmin;
-    sh_piv = pivmin;
-    sh_glo = glo - pad;
-    sh_ghi = ghi + pad;
-    sh_k0 = sturm_count(n, d, e2, pivmin, 0.0);
+  // Gershgorin bracket and pivmin (lane-parallel; max / min are order-independent)
+  const double el = l + 1 < n ? e[l] : 0.0;
+  const double e2l = el * el;
+  if (l + 1 < n) e2[l] = e2l;
+  const double pivmin = kPivminBase * fmax(1.0, wave_max(e2l));
+  double glo = INFINITY, ghi = -INFINITY;
+  if (l < n) {
+    const double r = (l > 0 ? fabs(e[l - 1]) : 0.0) + fabs(el);
+    glo = d[l] - r;
+    ghi = d[l] + r;
   }
+  glo = wave_min(glo);
+  ghi = wave_max(ghi);
+  const double pad = 2.0 * kEps * (fabs(glo) > fabs(ghi) ? fabs(glo) : fabs(ghi)) + pivmin;
+  glo -= pad;
+  ghi += pad;
   __syncthreads();
-  if (tid < 64) {                                          // wave 0
-    const int half = tid >> 5, mm = tid & 31;
-    const int k0 = sh_k0;
-    const int j1 = half == 0 ? 0 : n - 1;
-    const double r1 = sturm_find_half(n, d, e2, sh_piv, sh_glo, sh_ghi, j1, mm, half);
+  {
+    const int half = l >> 5, mm = l & 31;
+    int k0l = 0;
+    const double r1 = sturm_find_half(n, d, e2, pivmin, glo, ghi, half == 0 ? 0 : n - 1, mm, half, &k0l);
+    const int k0 = __builtin_amdgcn_readlane(k0l, 31);
     double r2 = 0.0;
     const bool inner = k0 > 0 && k0 < n;                   // wave-uniform
-    if (inner) r2 = sturm_find_half(n, d, e2, sh_piv, sh_glo, sh_ghi, half == 0 ? k0 - 1 : k0, mm, half);
+    if (inner) r2 = sturm_find_half(n, d, e2, pivmin, glo, ghi, half == 0 ? k0 - 1 : k0, mm, half, nullptr);
     if (mm == 0) {
       sh_res[half] = r1;
       sh_res[2 + half] = r2;
     }
+    if (l == 0) sh_k0 = k0;
   }
   __syncthreads();
-  if (tid == 0) {
+  if (l == 0) {
     const double lmin = sh_res[0], lmax = sh_res[1];
     const int k0 = sh_k0;
     double amin;
@@ -299,15 +310,20 @@ __global__ __launch_bounds__(256) void k_nnls_iterate(int64_t I, int R, int ldr,
                                                       const double *__restrict__ betas,
                                                       const double *__restrict__ M, const double *__restrict__ A,
                                                       double d1, double d2, int max_inner,
-                                                      unsigned long long *mask, double *__restrict__ hist) {
+                                                      unsigned long long *mask, double *__restrict__ hist,
+                                                      double *colsum) {
   __shared__ double sbeta[kMaxInner + 1];
+  if (blockIdx.x == 0 && (int)threadIdx.x < R) colsum[threadIdx.x] = 0.0;   // summed by k_nnls_select
   __shared__ double ybuf[4][RB];
+  __shared__ unsigned long long wbits[4][4];
   for (int k = threadIdx.x; k < max_inner; k += blockDim.x) sbeta[k] = betas[k];
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t row = (int64_t)blockIdx.x * 4 + wave;
-  if (row >= I) return;
+  unsigned long long bits[4] = {~0ull, ~0ull, ~0ull, ~0ull};
+  if (row < I) {
   const double lambda = prm->lambda, L = prm->L;
+  const double invL = 1.0 / L;
   const bool on = lane < R;
   const int r = on ? lane : 0;
   double z[RB];
@@ -316,51 +332,54 @@ __global__ __launch_bounds__(256) void k_nnls_iterate(int64_t I, int R, int ldr,
     const double vk = (on && k < R) ? V[k * R + r] : 0.0;
     z[k] = (k == r && on) ? vk + lambda : vk;
   }
+  // lanes >= R: zero start, zero column of Zt and zero Wt keep every iterate at +0
   const double a0 = on ? A[row * ldr + r] : 0.0;
   const double wt = on ? M[row * ldr + r] + lambda * a0 : 0.0;
   double y = a0, acur = a0;
   const int64_t hstride = I * ldr;
   double *hrow = hist + row * ldr + lane;
   if (lane < ldr) hrow[0] = a0;
-  unsigned long long bits[4] = {0ull, 0ull, 0ull, 0ull};
-  for (int k = 0; k < max_inner; ++k) {
-    if (lane < RB) ybuf[wave][lane] = y;
-    __builtin_amdgcn_wave_barrier();
-    double s4[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int kw = 0; kw < 4; ++kw) {
+    unsigned long long m = 0ull;
+    const int kend = max_inner - 64 * kw < 64 ? max_inner - 64 * kw : 64;
+    for (int kb = 0; kb < kend; ++kb) {
+      const int k = 64 * kw + kb;
+      const double beta = sbeta[k];
+      if (lane < RB) ybuf[wave][lane] = y;
+      __builtin_amdgcn_wave_barrier();
+      double s4[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int kk = 0; kk < RB; ++kk) s4[kk & 3] = fma(ybuf[wave][kk], z[kk], s4[kk & 3]);
-    __builtin_amdgcn_wave_barrier();
-    const double g = ((s4[0] + s4[1]) + (s4[2] + s4[3])) - wt;
-    const bool pass = !on || (!(fabs(g * y) > d1) && !(g < -d2));
-    if (pass) bits[k >> 6] |= 1ull << (k & 63);
-    const double tq = y - g / L;
-    const double an = tq > 0.0 ? tq : 0.0;
-    const double beta = sbeta[k];
-    y = on ? (1.0 + beta) * an - beta * acur : 0.0;
-    acur = an;
-    if (lane < ldr) hrow[(k + 1) * hstride] = on ? acur : 0.0;
-  }
-  for (int w = 0; w < 4; ++w) {
-    unsigned long long m = bits[w];
-    for (int o = 32; o > 0; o >>= 1) m &= __shfl_xor(m, o, 64);
-    bits[w] = m;
-  }
-  if (lane == 0) {
-    for (int w = 0; w < 4; ++w) {
-      unsigned long long m = bits[w];
-      const int lo = w * 64;
-      if (max_inner < lo + 64) m |= (max_inner <= lo) ? ~0ull : (~0ull << (max_inner - lo));
-      atomicAnd(&mask[w], m);
+      for (int kk = 0; kk < RB; ++kk) s4[kk & 3] = fma(ybuf[wave][kk], z[kk], s4[kk & 3]);
+      __builtin_amdgcn_wave_barrier();
+      const double g = ((s4[0] + s4[1]) + (s4[2] + s4[3])) - wt;
+      const bool fail = on && ((fabs(g * y) > d1) || (g < -d2));
+      if (__ballot(fail) == 0ull) m |= 1ull << kb;       // the row passes check k
+      // g / L as RN(g / L): reciprocal, then one exact-residual correction (Markstein)
+      const double q0 = g * invL;
+      const double tq = y - fma(fma(-q0, L, g), invL, q0);
+      const double an = tq > 0.0 ? tq : 0.0;
+      y = (1.0 + beta) * an - beta * acur;
+      acur = an;
+      if (lane < ldr) hrow[(k + 1) * hstride] = acur;
     }
+    bits[kw] = m;
+  }
+  }
+  // the block's mask (AND over its rows) goes to its own slot: no atomics
+  if (lane == 0)
+    for (int w = 0; w < 4; ++w) wbits[wave][w] = bits[w];
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    const int w = threadIdx.x;
+    unsigned long long m = ((wbits[0][w] & wbits[1][w]) & wbits[2][w]) & wbits[3][w];
+    const int lo = w * 64;
+    if (max_inner < lo + 64) m |= (max_inner <= lo) ? ~0ull : (~0ull << (max_inner - lo));
+    mask[(int64_t)blockIdx.x * 4 + w] = m;
   }
 }
 
-// Momentum coefficients beta_k of NesterovMNLS.hpp:222-223 (shared by every row), and
-// the reset of the satisfaction mask and the column sums.
-__global__ void k_nnls_prep(const NnlsParams *prm, int max_inner, double *betas, unsigned long long *mask,
-                            double *colsum, int R) {
-  if (threadIdx.x < 4) mask[threadIdx.x] = ~0ull;
-  if ((int)threadIdx.x < R) colsum[threadIdx.x] = 0.0;
+// Momentum coefficients beta_k of NesterovMNLS.hpp:222-223 (shared by every row).
+__global__ void k_nnls_prep(const NnlsParams *prm, int max_inner, double *betas) {
   if (threadIdx.x == 0) {
     const double q = prm->q;
     double alpha = 1.0;
@@ -372,7 +391,7 @@ __global__ void k_nnls_prep(const NnlsParams *prm, int max_inner, double *betas,
   }
 }
 
-__device__ __forceinline__ int first_common_check(const unsigned long long *mask, int max_inner) {
+__device__ __forceinline__ int first_common_check(const volatile unsigned long long *mask, int max_inner) {
   for (int w = 0; w < 4; ++w) {
     unsigned long long m = mask[w];
     if (m) {
@@ -390,7 +409,23 @@ __global__ __launch_bounds__(256) void k_nnls_select(int64_t I, int R, int ldr, 
                                                      const unsigned long long *__restrict__ mask, int max_inner,
                                                      double *__restrict__ Anew, double *colsum, NnlsParams *prm) {
   __shared__ double red[4][kMaxRank];
-  const int kstar = first_common_check(mask, max_inner);
+  __shared__ unsigned long long mred[256];
+  __shared__ int sk;
+  {
+    // AND of the per-block masks of k_nnls_iterate (4 words per block)
+    const int64_t nblk = (I + 3) / 4;
+    unsigned long long m = ~0ull;
+    for (int64_t b = threadIdx.x >> 2; b < nblk; b += 64) m &= mask[b * 4 + (threadIdx.x & 3)];
+    mred[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 128; o >= 4; o >>= 1) {
+      if ((int)threadIdx.x < o) mred[threadIdx.x] &= mred[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) sk = first_common_check(mred, max_inner);
+    __syncthreads();
+  }
+  const int kstar = sk;
   if (blockIdx.x == 0 && threadIdx.x == 0) prm->iters = kstar;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const double *src = hist + (int64_t)kstar * I * ldr;
@@ -464,11 +499,12 @@ struct NormArgs {
   double *A[kMaxOrder];
 };
 
+// Column norms, scales and the Gram rescaling of Normalize (one block).  Columns whose
+// norms vanish keep scale 1 on every factor, so the row pass leaves them bit-identical.
 __global__ __launch_bounds__(256) void k_normalize(NormArgs na, double *scratch) {
   const int N = na.N, R = na.R, last = N - 1;
   double *lam = scratch;                       // (N-1) x R
   double *scale = scratch + kMaxOrder * kMaxRank;  // N x R column scales (1 = untouched)
-  __shared__ int nonzero[kMaxRank];
   if (threadIdx.x < R) {
     const int i = threadIdx.x;
     int pass = 0;
@@ -483,7 +519,6 @@ __global__ __launch_bounds__(256) void k_normalize(NormArgs na, double *scratch)
       nf[j] = sqrt(lam[j * R + i]);
       nz = !(nf[j] == 0.0);
     }
-    nonzero[i] = nz;
     if (!nz) {
       for (int j = 0; j < last; ++j) lam[j * R + i] = 1.0;
     } else {
@@ -499,15 +534,6 @@ __global__ __launch_bounds__(256) void k_normalize(NormArgs na, double *scratch)
     }
   }
   __syncthreads();
-  {
-    const int c = threadIdx.x & 63;
-    for (int k = 0; k < N; ++k) {
-      if ((k == last && k != na.wf) || c >= R || !nonzero[c]) continue;
-      const double sc = scale[k * R + c];
-      for (int64_t row = threadIdx.x >> 6; row < na.dims[k]; row += blockDim.x >> 6)
-        na.A[k][row * na.ldr + c] = na.A[k][row * na.ldr + c] * sc;
-    }
-  }
   for (int e = threadIdx.x; e < R * R; e += blockDim.x) {
     int a = e / R, b = e % R;
     double nmw = 1.0;
@@ -521,6 +547,16 @@ __global__ __launch_bounds__(256) void k_normalize(NormArgs na, double *scratch)
     }
     na.G[na.wf][e] = na.G[na.wf][e] * nmw;
   }
+}
+
+// Row pass of Normalize: A_k[row][c] *= scale[k][c]; block (row tile of 4, factor k).
+__global__ __launch_bounds__(256) void k_normalize_rows(NormArgs na, const double *__restrict__ scratch) {
+  const int k = blockIdx.y, c = threadIdx.x & 63;
+  if ((k == na.N - 1 && k != na.wf) || c >= na.R) return;
+  const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= na.dims[k]) return;
+  const double sc = scratch[kMaxOrder * kMaxRank + k * na.R + c];
+  na.A[k][row * na.ldr + c] = na.A[k][row * na.ldr + c] * sc;
 }
 
 __global__ void k_axpy_accel(int64_t n, double step, const double *old, const double *cur, double *out) {
@@ -550,24 +586,26 @@ void gram(hipStream_t s, int64_t I, int R, int ldr, const double *A, double *G, 
 }
 
 void eig_extremes(hipStream_t s, int R, const double *V, NnlsParams *p) {
-  if (R <= 16) k_eig_extremes<16><<<1, 256, 0, s>>>(R, V, p);
-  else if (R <= 32) k_eig_extremes<32><<<1, 256, 0, s>>>(R, V, p);
-  else k_eig_extremes<64><<<1, 256, 0, s>>>(R, V, p);
+  if (R <= 16) k_eig_extremes<16><<<1, 64, 0, s>>>(R, V, p);
+  else if (R <= 32) k_eig_extremes<32><<<1, 64, 0, s>>>(R, V, p);
+  else k_eig_extremes<64><<<1, 64, 0, s>>>(R, V, p);
 }
 
-void nnls_prep(hipStream_t s, const NnlsParams *p, int max_inner, double *betas, uint64_t *mask, double *colsum,
-               int R) {
-  k_nnls_prep<<<1, 64, 0, s>>>(p, max_inner, betas, (unsigned long long *)mask, colsum, R);
+void nnls_prep(hipStream_t s, const NnlsParams *p, int max_inner, double *betas) {
+  k_nnls_prep<<<1, 64, 0, s>>>(p, max_inner, betas);
 }
 
 void nnls_iterate(hipStream_t s, int64_t I, int R, int ldr, const double *V, const NnlsParams *p,
                   const double *betas, const double *M, const double *A, double d1, double d2, int max_inner,
-                  uint64_t *mask, double *hist) {
+                  uint64_t *mask, double *hist, double *colsum) {
   const unsigned grid = (unsigned)((I + 3) / 4);
   unsigned long long *m = (unsigned long long *)mask;
-  if (R <= 16) k_nnls_iterate<16><<<grid, 256, 0, s>>>(I, R, ldr, V, p, betas, M, A, d1, d2, max_inner, m, hist);
-  else if (R <= 32) k_nnls_iterate<32><<<grid, 256, 0, s>>>(I, R, ldr, V, p, betas, M, A, d1, d2, max_inner, m, hist);
-  else k_nnls_iterate<64><<<grid, 256, 0, s>>>(I, R, ldr, V, p, betas, M, A, d1, d2, max_inner, m, hist);
+#define PTN_NI(RBV) \
+  k_nnls_iterate<RBV><<<grid, 256, 0, s>>>(I, R, ldr, V, p, betas, M, A, d1, d2, max_inner, m, hist, colsum)
+  if (R <= 16) PTN_NI(16);
+  else if (R <= 32) PTN_NI(32);
+  else PTN_NI(64);
+#undef PTN_NI
 }
 
 void nnls_select(hipStream_t s, int64_t I, int R, int ldr, const double *hist, const uint64_t *mask, int max_inner,
@@ -593,7 +631,10 @@ void normalize(hipStream_t s, int N, const int64_t *dims, int R, int ldr, int wf
   NormArgs na{};
   na.N = N; na.R = R; na.ldr = ldr; na.wf = wf;
   for (int k = 0; k < N; ++k) { na.dims[k] = dims[k]; na.G[k] = grams[k]; na.A[k] = fac[k]; }
+  int64_t maxI = 0;
+  for (int k = 0; k < N; ++k) maxI = dims[k] > maxI ? dims[k] : maxI;
   k_normalize<<<1, 256, 0, s>>>(na, scratch);
+  k_normalize_rows<<<dim3((unsigned)((maxI + 3) / 4), (unsigned)N), 256, 0, s>>>(na, scratch);
 }
 
 void axpy_accel(hipStream_t s, int64_t n, double step, const double *old, const double *cur, double *out) {

@@ -223,6 +223,55 @@ def test_als_small_parity(h, dims, R):
         _compare_runs(Fg, stg, logg, Fo, sto, logo, 1e-10)
 
 
+@pytest.mark.parametrize("arith", [P.ARITH_F32, P.ARITH_BF16X3])
+@pytest.mark.parametrize("dims,R", [([100, 100, 100], 10), ([24, 20, 18, 16], 7)])
+def test_als_fp32_stream_and_fused_line_search(h, arith, dims, R):
+    """FP32 tensor stream on the dimension tree, with and without the line-search pass
+    fused into the next left node.  The paired pass computes the lower half of its
+    output column by column exactly as the single left node does, so while the two runs
+    take the same acceleration decisions they are bitwise equal apart from f_accel
+    (whose fit term is summed in another order).  Against the FP64 oracle the final fit
+    agrees to the FP32 stream's accuracy."""
+    say("ALS fp32 stream", dims, R, "arith", arith)
+    X, _ = o.make_tensor(dims, R, seed=0, snr_db=30.0)
+    init = o.gen_factors(dims, R, seed=1)
+    h.set_tensor(X, storage=P.TENSOR_BOTH)
+    kw = dict(max_iter=25, threshold_error=0.0, mttkrp_form=P.MTTKRP_DIMTREE, arith=arith)
+    F0, st0, lg0 = h.cpd(R, init, fuse_accel=0, **kw)
+    F1, st1, lg1 = h.cpd(R, init, fuse_accel=1, **kw)
+    Fo, sto, logo = o.cpd(X, R, init, mttkrp_form=o.MTTKRP_DIMTREE, cost_form=o.COST_GRAM, max_iter=25,
+                          threshold_error=0.0)
+    acc0 = [l["accel_accepted"] for l in lg0]
+    acc1 = [l["accel_accepted"] for l in lg1]
+    say("accept unfused", acc0, "fused", acc1, "oracle", [l["accel_accepted"] for l in logo])
+    tried = [i for i, l in enumerate(lg0) if l["accel_tried"]]
+    assert tried and [i for i, l in enumerate(lg1) if l["accel_tried"]] == tried
+    # first line search: same state in both runs, the fit term <X, [[Ahat]]> summed in
+    # another order; FP32 accumulation bounds its difference by ~1e-6 of ||X||^2
+    frob = float(np.sum(X * X))
+    i = tried[0]
+    fa0, fa1 = lg0[i]["f_accel"], lg1[i]["f_accel"]
+    say("first f_accel", fa0, fa1, "||X||^2", frob)
+    assert abs(fa1 * fa1 - fa0 * fa0) < 1e-5 * frob
+    if acc0 == acc1:
+        assert [l["nnls_iters"] for l in lg0] == [l["nnls_iters"] for l in lg1]
+        for a, b in zip(F0, F1):
+            assert np.array_equal(a, b)
+    # against the FP64 oracle: FP32 errors scale with ||X||^2, so squared residuals are
+    # compared on that scale.  The per-sweep cost tracks the oracle's until the first
+    # acceleration decision that a near tie sends the other way; afterwards the runs
+    # follow different (equally valid) ALS paths and only the final fit is compared.
+    acco = [l["accel_accepted"] for l in logo]
+    rel_o = o.cost_explicit(X, Fo) / np.sqrt(frob)
+    for F, lg, acc in ((F0, lg0, acc0), (F1, lg1, acc1)):
+        dvg = next((i for i, (a, b) in enumerate(zip(acc, acco)) if a != b), len(acco))
+        for i in range(dvg):
+            assert abs(lg[i]["f_value"] ** 2 - logo[i]["f_value"] ** 2) < 1e-5 * frob, i
+        rel = h.cost_explicit(F) / np.sqrt(frob)
+        say("same decisions as the oracle for", dvg, "sweeps; final rel cost", rel, "oracle", rel_o)
+        assert abs(rel * rel - rel_o * rel_o) < 1e-5
+
+
 def test_zero_column_fallback_on_gpu(h):
     say("zero-column fallback")
     rng = np.random.default_rng(12)
@@ -247,3 +296,8 @@ def test_invalid_arguments_raise(h):
         h.mttkrp(init, 5)                                           # bad mode
     with pytest.raises(P.PtnError):
         h.mttkrp(init, 0, arith=P.ARITH_F32_F64)                    # no FP32 copy uploaded
+    h.set_tensor(np.ones((4, 5, 6), order="F"), storage=P.TENSOR_BOTH)
+    with pytest.raises(P.PtnError):
+        h.cpd(3, init, fuse_accel=1, arith=P.ARITH_F32)             # paired pass needs the tree
+    with pytest.raises(P.PtnError):
+        h.cpd(3, init, fuse_accel=1, mttkrp_form=P.MTTKRP_DIMTREE)  # ... and FP32-output contractions
