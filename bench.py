@@ -46,9 +46,27 @@ def timed_sweeps(status, sweep_log, W, K):
     return t_end[W + K - 1] - t0
 
 
-def cpu_baseline(seconds_budget=30.0):
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_left_node_800cubed_R32.md")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the left-node contraction from the committed counter
+    passes (FETCH_SIZE and WRITE_SIZE, scaled by a plain copy's readings)."""
+    try:
+        vals = {}
+        for line in open(TRAFFIC_FILE):
+            k, sep, v = line.partition(":")
+            if sep and k.strip() in ("left_node_traffic_bytes",):
+                vals[k.strip()] = float(v)
+        return vals.get("left_node_traffic_bytes")
+    except OSError:
+        return None
+
+
+def cpu_baseline(sweeps=8):
     """FP64 OpenMP restatement of cpd_omp (CpdOpenMP.hpp:266-329, blocked MTTKRP
-    MTTKRP.hpp:102-225, explicit cost) on the same 800^3 workload, 2 sweeps."""
+    MTTKRP.hpp:102-225, explicit cost) on the same 800^3 workload, a bounded number
+    of sweeps (about 10 s on 16 cores)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as o
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
@@ -56,14 +74,32 @@ def cpu_baseline(seconds_budget=30.0):
     X, _ = o.make_tensor(DIMS, R, seed=0, snr_db=20.0)
     init = o.gen_factors(DIMS, R, seed=1)
     log("cpu baseline: tensor generated in %.1f s, %d threads" % (time.time() - t, threads))
-    sweeps = 2
     F, st, lg = o.cpd(X, R, init, max_iter=sweeps, threshold_error=0.0, mttkrp_form=o.MTTKRP_BLOCKED,
                       cost_form=o.COST_EXPLICIT, threads=threads)
     secs = lg[-1]["t_end"] - lg[0]["t_start"]
     del X
     return {"value": sweeps / secs, "unit": "sweeps/s", "cores": threads, "kind": "port",
-            "sample": "2 full sweeps (incl. one acceleration step) of the 800^3 R=32 FP64 workload, "
-                      "oracle cpd_omp restatement: blocked MTTKRP + explicit cost, %.1f s" % secs}
+            "sample": "%d full sweeps (acceleration from the second) of the 800^3 R=32 FP64 workload, "
+                      "oracle cpd_omp restatement: blocked MTTKRP + explicit cost, %.1f s" % (sweeps, secs)}
+
+
+def copy_calibration(nbytes):
+    """Device-to-device hipMemcpy of nbytes, three times: the plain copy kernel whose
+    FETCH_SIZE / WRITE_SIZE counters calibrate the HBM traffic of the contraction."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, ctypes.c_size_t(nbytes))
+    for _ in range(3):
+        if hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) != 0:    # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log("copy calibration: 3 device copies of %d bytes" % nbytes)
 
 
 def main():
@@ -74,7 +110,12 @@ def main():
     ap.add_argument("--arith", default="bf16x3", choices=["bf16x3", "f32", "f32_f64"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 GPU run")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device copy of BYTES (HBM counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
@@ -153,7 +194,7 @@ def main():
             "mttkrp_gbs": {"left_node": gbs_l, "right_node": gbs_r},
         },
         "roofline": {"bound": "hbm", "achieved": gbs_l, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": gbs_l / HBM_PEAK_GBS, "traffic": None,
+                     "frac": gbs_l / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "kernel": "left-node contraction X x_2 A_2 (k_gemm_cm)", "avg_ms": ms_l,
                      "algorithmic_bytes": bytes_l},
         "cpu_baseline": cpu,
