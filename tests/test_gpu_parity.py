@@ -272,6 +272,25 @@ def test_als_fp32_stream_and_fused_line_search(h, arith, dims, R):
         assert abs(rel * rel - rel_o * rel_o) < 1e-5
 
 
+@pytest.mark.parametrize("dims,R,seed,sweeps", [([5, 4, 6, 3, 4], 3, 5, 9), ([4, 3, 5, 3, 4, 2], 3, 6, 10),
+                                                ([3, 4, 5, 3, 2, 3], 2, 7, 10)])
+def test_als_higher_order_standard_form(h, dims, R, seed, sweeps):
+    """Orders 5 and 6 through the per-mode form (the tree is built for 3 and 4).  The
+    sweep counts stop before the runs converge to where f and the accelerated cost tie
+    to ~1e-14 and the decision rests on the last bits (smallest oracle margin here
+    >= 1.4e-4 relative)."""
+    say("ALS order", len(dims), dims, R)
+    X, _ = o.make_tensor(dims, R, seed=seed, snr_db=25.0)
+    init = o.gen_factors(dims, R, seed=seed + 1)
+    h.set_tensor(X)
+    kw = dict(max_iter=sweeps, threshold_error=0.0)
+    Fg, stg, logg = h.cpd(R, init, **kw)
+    Fo, sto, logo = o.cpd(X, R, init, mttkrp_form=o.MTTKRP_STANDARD, cost_form=o.COST_GRAM, **kw)
+    _compare_runs(Fg, stg, logg, Fo, sto, logo, 1e-10)
+    with pytest.raises(P.PtnError):
+        h.cpd(R, init, mttkrp_form=P.MTTKRP_DIMTREE, **kw)
+
+
 def test_zero_column_fallback_on_gpu(h):
     say("zero-column fallback")
     rng = np.random.default_rng(12)
