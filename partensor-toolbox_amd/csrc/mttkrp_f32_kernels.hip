@@ -107,8 +107,12 @@ __device__ __forceinline__ void mma_step(const float (&x)[8], const bf16x8 &bhi,
 
 // ---------------------------------------------------------------------------
 // gemm_cm_f32: each wave owns 64 consecutive rows (2 tiles of 32) and the block's
-// k-range; tensor loads for the next 16-wide k step are issued before the current
-// step's conversion and MFMAs.
+// k-range.  Each 16-wide k step loads its operand fragments and the 2 x 8 tensor
+// values of every lane (unconditional loads from clamped addresses), converts and
+// multiplies; occupancy hides the load latency.  The 64 x RPF result leaves through
+// LDS, one 32 x 32 tile at a time, as 16-byte non-temporal row stores: scattered
+// 4-byte stores of the MFMA layout cost several times their bytes next to the
+// tensor stream.
 // ---------------------------------------------------------------------------
 template <int MODE, int NT>
 __global__ __launch_bounds__(256) void k_gemm_cm_f32(const float *__restrict__ X, int64_t M, int64_t K,
@@ -116,10 +120,11 @@ __global__ __launch_bounds__(256) void k_gemm_cm_f32(const float *__restrict__ X
                                                      const bf16x8 *__restrict__ wlo, const float *__restrict__ wf,
                                                      float *__restrict__ out) {
   constexpr int RPF = 32 * NT;
+  __shared__ float stg[4][32 * 33];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int i = lane & 31, kg = lane >> 5;
   const int64_t mb = ((int64_t)blockIdx.x * 4 + wave) * 64;
-  if (mb >= M) return;
+  if (mb >= M) return;                                   // no block-wide barrier below
   const int64_t kbeg = (int64_t)blockIdx.y * kchunk;
   const int64_t kend = kbeg + kchunk < K ? kbeg + kchunk : K;
 
@@ -131,29 +136,13 @@ __global__ __launch_bounds__(256) void k_gemm_cm_f32(const float *__restrict__ X
 #pragma unroll
       for (int v = 0; v < 16; ++v) acc[s][t][v] = 0.f;
 
-  // Loads are unconditional from clamped (always valid) addresses and nothing is
-  // zeroed: a missing row only fills its own (discarded) row of D, and k >= K meets
-  // the zero rows of the packed operand.  Splits are multiples of 16 columns, so a
-  // step never crosses into the next split.
+  // A missing row only fills its own (discarded) row of D, and k >= K meets the zero
+  // rows of the packed operand.  Splits are multiples of 16 columns, so a step never
+  // crosses into the next split.
   const float *p0 = X + (mb + i < M ? mb + i : M - 1);
   const float *p1 = X + (mb + 32 + i < M ? mb + 32 + i : M - 1);
-  float xn[2][8];
-  auto load = [&](int64_t k0, float (&dst)[2][8]) {
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int64_t k = k0 + 8 * kg + j;
-      const int64_t kc = k < K ? k : K - 1;
-      dst[0][j] = __builtin_nontemporal_load(p0 + M * kc);
-      dst[1][j] = __builtin_nontemporal_load(p1 + M * kc);
-    }
-  };
-  if (kbeg < kend) load(kbeg, xn);
   for (int64_t k0 = kbeg; k0 < kend; k0 += 16) {
-    float xc[2][8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) { xc[0][j] = xn[0][j]; xc[1][j] = xn[1][j]; }
     const int64_t ks = k0 >> 4;
-    // this step's packed operand before the next step's tensor loads (in-order counters)
     bf16x8 bh[NT], bl[NT];
     if constexpr (MODE == kGemmBf16x3) {
 #pragma unroll
@@ -162,7 +151,14 @@ __global__ __launch_bounds__(256) void k_gemm_cm_f32(const float *__restrict__ X
         bl[t] = wlo[(ks * NT + t) * 64 + lane];
       }
     }
-    load(k0 + 16, xn);                       // clamped: always valid, one path through the loop
+    float xc[2][8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int64_t k = k0 + 8 * kg + j;
+      const int64_t kc = k < K ? k : K - 1;
+      xc[0][j] = __builtin_nontemporal_load(p0 + M * kc);
+      xc[1][j] = __builtin_nontemporal_load(p1 + M * kc);
+    }
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
       const float *wrow = wf ? wf + k0 * RPF : nullptr;
@@ -171,15 +167,23 @@ __global__ __launch_bounds__(256) void k_gemm_cm_f32(const float *__restrict__ X
     }
   }
   float *o = out + (int64_t)blockIdx.y * M * RPF;
+  float *st = stg[wave];
 #pragma unroll
   for (int s = 0; s < 2; ++s)
 #pragma unroll
-    for (int v = 0; v < 16; ++v) {
-      const int64_t row = mb + 32 * s + 8 * (v / 4) + 4 * kg + (v % 4);
-      if (row < M) {
+    for (int t = 0; t < NT; ++t) {
 #pragma unroll
-        for (int t = 0; t < NT; ++t) o[row * RPF + 32 * t + i] = acc[s][t][v];
+      for (int v = 0; v < 16; ++v) st[(8 * (v / 4) + 4 * kg + (v % 4)) * 33 + i] = acc[s][t][v];
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int e = 64 * q + lane;                     // 16-byte chunk of the 32 x 32 tile
+        const int r = e >> 3, c4 = 4 * (e & 7);
+        const int64_t row = mb + 32 * s + r;
+        const f32x4 val = {st[r * 33 + c4], st[r * 33 + c4 + 1], st[r * 33 + c4 + 2], st[r * 33 + c4 + 3]};
+        if (row < M) __builtin_nontemporal_store(val, (f32x4 *)(o + row * RPF + 32 * t + c4));
       }
+      __builtin_amdgcn_wave_barrier();
     }
 }
 

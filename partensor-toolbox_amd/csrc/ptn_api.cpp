@@ -430,9 +430,7 @@ void prep_mode(Work &w, const double *gbase, int mode, bool tree, const ModeIn &
   int n = v_order(w.N, mode, tree, list);
   const double *gp[kMaxOrder];
   for (int j = 0; j < n; ++j) gp[j] = gbase + (int64_t)list[j] * w.R * w.R;
-  hadamard(s, w.R, n, gp, in.V);
-  eig_extremes(s, w.R, in.V, in.p);
-  nnls_prep(s, in.p, max_inner, in.betas);
+  nnls_setup(s, w.R, n, gp, in.V, in.p, max_inner, in.betas);
 }
 
 void gram_of(Work &w, const double *A, int64_t I, double *G) { gram(w.h->stream, I, w.R, w.ldr, A, G, w.gpart); }
@@ -506,11 +504,11 @@ void update_nonneg(Work &w, int mode, const ptn_options &o, const ModeIn &in) {
     w.hist.alloc(h, need);
     w.hist_cap = need;
   }
-  HIPCHK(hipMemcpyAsync(w.Aold, A, I * w.ldr * sizeof(double), hipMemcpyDeviceToDevice, s));
-  nnls_iterate(s, I, w.R, w.ldr, in.V, p, in.betas, w.M, w.Aold, o.nesterov_delta_1, o.nesterov_delta_2,
-               o.max_inner, w.mask, w.hist, w.colsum);
+  // the start point A_old stays in hist[0] (written by the iterate kernel) for the fallback
+  nnls_iterate(s, I, w.R, w.ldr, in.V, p, in.betas, w.M, A, o.nesterov_delta_1, o.nesterov_delta_2, o.max_inner,
+               w.mask, w.hist, w.colsum);
   nnls_select(s, I, w.R, w.ldr, w.hist, w.mask, o.max_inner, A, w.colsum, p);
-  zero_col_fallback(s, I, w.R, w.ldr, w.colsum, w.Aold, A, p);
+  zero_col_fallback(s, I, w.R, w.ldr, w.colsum, w.hist, A, p);
 }
 
 }  // namespace

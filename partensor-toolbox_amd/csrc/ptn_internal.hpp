@@ -94,6 +94,10 @@ void hadamard(hipStream_t s, int R, int ng, const double *const *grams, double *
 // partials: ceil(I/32) x R x R doubles
 void gram(hipStream_t s, int64_t I, int R, int ldr, const double *A, double *G, double *partials);
 void eig_extremes(hipStream_t s, int R, const double *V, NnlsParams *p);
+// V = Hadamard of the ng Grams (list order), its extreme eigenvalues with GLambda, and
+// the max_inner momentum coefficients: everything an NNLS update needs besides M.
+void nnls_setup(hipStream_t s, int R, int ng, const double *const *grams, double *V, NnlsParams *p, int max_inner,
+                double *betas);
 // betas: max_inner doubles
 void nnls_prep(hipStream_t s, const NnlsParams *p, int max_inner, double *betas);
 // satisfaction masks of nnls_iterate: 4 words per block of 4 rows

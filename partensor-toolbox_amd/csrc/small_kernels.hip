@@ -89,7 +89,7 @@ __global__ __launch_bounds__(256) void k_gram_chunks(int64_t I, int R, int ldr, 
 
 __global__ __launch_bounds__(256) void k_gram_combine(int R, int nchunks, const double *__restrict__ partials,
                                                       double *__restrict__ G) {
-  for (int e = threadIdx.x; e < R * R; e += blockDim.x) {
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < R * R; e += gridDim.x * blockDim.x) {
     double g = partials[e];
     int c = 1;
     for (; c + 4 <= nchunks; c += 4) {
@@ -115,6 +115,14 @@ __device__ int sturm_count(int n, const double *d, const double *e2, double pivm
     if (q < 0.0) ++c;
   }
   return c;
+}
+
+__device__ __forceinline__ double update_alpha(double alpha, double q) {
+  double a = 1.0;
+  double b = alpha * alpha - q;
+  double c = -alpha * alpha;
+  double D = b * b - 4.0 * a * c;
+  return (-b + sqrt(D)) / 2.0;
 }
 
 // Multisection for the j-th smallest eigenvalue, run by one 32-lane half of the wave:
@@ -178,8 +186,14 @@ __device__ __forceinline__ double tree64(double x) {
 // Sums: 64-lane trees (tree64) for sigma and p.v, four interleaved chains over the RB
 // padded rows for the matrix-vector product; the CPU restatement performs the same
 // operations in the same order, padding included.
+// With ng > 0 the matrix is first formed as the Hadamard product of the ng Grams in
+// list order (V = G_l0, V = G_lj * V; PartialCwiseProd) and stored to Vout; with
+// betas != nullptr the momentum coefficients of the Nesterov solve follow the
+// eigenvalues (one kernel prepares an NNLS update).
 template <int RB>
-__global__ __launch_bounds__(64) void k_eig_extremes(int R, const double *__restrict__ V, NnlsParams *prm) {
+__global__ __launch_bounds__(64) void k_eig_extremes(int R, const double *__restrict__ V, GramPtrs G, int ng,
+                                                     double *__restrict__ Vout, NnlsParams *prm, int max_inner,
+                                                     double *__restrict__ betas) {
   __shared__ double t[RB * RB];
   __shared__ double vb[RB], pb[RB], d[RB], e[RB], e2[RB];
   __shared__ double sh_res[4];
@@ -187,7 +201,18 @@ __global__ __launch_bounds__(64) void k_eig_extremes(int R, const double *__rest
   const int n = R, l = threadIdx.x;
   for (int x = l; x < RB * RB; x += 64) {
     const int r = x / RB, c = x % RB;
-    t[x] = (r < n && c < n) ? V[r * n + c] : 0.0;
+    double v = 0.0;
+    if (r < n && c < n) {
+      const int e = r * n + c;
+      if (ng > 0) {
+        v = G.p[0][e];
+        for (int j = 1; j < ng; ++j) v = G.p[j][e] * v;
+        Vout[e] = v;
+      } else {
+        v = V[e];
+      }
+    }
+    t[x] = v;
   }
   if (l < RB) { e[l] = 0.0; vb[l] = 0.0; pb[l] = 0.0; }
   __syncthreads();
@@ -284,16 +309,17 @@ __global__ __launch_bounds__(64) void k_eig_extremes(int R, const double *__rest
     prm->L = L;
     prm->lambda = lambda;
     prm->q = q;
+    if (betas) {                                   // NesterovMNLS.hpp:222-223
+      double alpha = 1.0;
+      for (int k = 0; k < max_inner; ++k) {
+        const double na = update_alpha(alpha, q);
+        betas[k] = alpha * (1.0 - alpha) / (alpha * alpha + na);
+        alpha = na;
+      }
+    }
   }
 }
 
-__device__ __forceinline__ double update_alpha(double alpha, double q) {
-  double a = 1.0;
-  double b = alpha * alpha - q;
-  double c = -alpha * alpha;
-  double D = b * b - 4.0 * a * c;
-  return (-b + sqrt(D)) / 2.0;
-}
 
 // One 64-lane wave per factor row; lane r holds column r and keeps column r of
 // Zt = V + lambda I in registers.  Each step the row y is broadcast through LDS and
@@ -582,13 +608,23 @@ void hadamard(hipStream_t s, int R, int ng, const double *const *grams, double *
 void gram(hipStream_t s, int64_t I, int R, int ldr, const double *A, double *G, double *partials) {
   const int nchunks = (int)((I + 31) / 32);
   k_gram_chunks<<<nchunks, 256, 0, s>>>(I, R, ldr, A, partials);
-  k_gram_combine<<<1, 256, 0, s>>>(R, nchunks, partials, G);
+  k_gram_combine<<<(unsigned)((R * R + 63) / 64), 64, 0, s>>>(R, nchunks, partials, G);
 }
 
 void eig_extremes(hipStream_t s, int R, const double *V, NnlsParams *p) {
-  if (R <= 16) k_eig_extremes<16><<<1, 64, 0, s>>>(R, V, p);
-  else if (R <= 32) k_eig_extremes<32><<<1, 64, 0, s>>>(R, V, p);
-  else k_eig_extremes<64><<<1, 64, 0, s>>>(R, V, p);
+  const GramPtrs G{};
+  if (R <= 16) k_eig_extremes<16><<<1, 64, 0, s>>>(R, V, G, 0, nullptr, p, 0, nullptr);
+  else if (R <= 32) k_eig_extremes<32><<<1, 64, 0, s>>>(R, V, G, 0, nullptr, p, 0, nullptr);
+  else k_eig_extremes<64><<<1, 64, 0, s>>>(R, V, G, 0, nullptr, p, 0, nullptr);
+}
+
+void nnls_setup(hipStream_t s, int R, int ng, const double *const *grams, double *V, NnlsParams *p, int max_inner,
+                double *betas) {
+  GramPtrs G{};
+  for (int j = 0; j < ng; ++j) G.p[j] = grams[j];
+  if (R <= 16) k_eig_extremes<16><<<1, 64, 0, s>>>(R, nullptr, G, ng, V, p, max_inner, betas);
+  else if (R <= 32) k_eig_extremes<32><<<1, 64, 0, s>>>(R, nullptr, G, ng, V, p, max_inner, betas);
+  else k_eig_extremes<64><<<1, 64, 0, s>>>(R, nullptr, G, ng, V, p, max_inner, betas);
 }
 
 void nnls_prep(hipStream_t s, const NnlsParams *p, int max_inner, double *betas) {
