@@ -258,7 +258,9 @@ __global__ __launch_bounds__(256) void k_contract_rows(const TT *__restrict__ T,
 template <typename TT>
 __global__ __launch_bounds__(256) void k_contract2(const TT *__restrict__ Y, int64_t ld, int64_t D0, int64_t D1,
                                                    int outmode, const double *__restrict__ F, int ldr, int R,
-                                                   int64_t slen, double *__restrict__ part) {
+                                                   int64_t slen, double *__restrict__ part, const int *sel,
+                                                   int64_t sel_off) {
+  if (sel) Y += (int64_t)(*sel) * sel_off;              // node chosen on the device (line search)
   __shared__ double red[8][32];
   const int q = threadIdx.x >> 5, c0 = threadIdx.x & 31;
   const int NT = (R + 31) / 32;
@@ -378,12 +380,12 @@ void contract_rows(hipStream_t s, const double *T64, const float *T32, int64_t l
 }
 
 void contract2(hipStream_t s, const double *Y64, const float *Y32, int64_t ld, int64_t D0, int64_t D1, int outmode,
-               const double *F, int ldr, int R, int splits, double *part) {
+               const double *F, int ldr, int R, int splits, double *part, const int *sel, int64_t sel_off) {
   const int64_t D = outmode == 0 ? D1 : D0;             // contracted extent
   const int64_t slen = (D + splits - 1) / splits;
   dim3 grid(outmode == 0 ? (unsigned)((D0 + 7) / 8) : (unsigned)D1, (unsigned)splits);
-  if (Y64) k_contract2<double><<<grid, 256, 0, s>>>(Y64, ld, D0, D1, outmode, F, ldr, R, slen, part);
-  else     k_contract2<float><<<grid, 256, 0, s>>>(Y32, ld, D0, D1, outmode, F, ldr, R, slen, part);
+  if (Y64) k_contract2<double><<<grid, 256, 0, s>>>(Y64, ld, D0, D1, outmode, F, ldr, R, slen, part, sel, sel_off);
+  else     k_contract2<float><<<grid, 256, 0, s>>>(Y32, ld, D0, D1, outmode, F, ldr, R, slen, part, sel, sel_off);
 }
 
 void reduce_splits(hipStream_t s, const double *partials, int splits, int64_t n, double *out) {

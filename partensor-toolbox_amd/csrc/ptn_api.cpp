@@ -82,6 +82,20 @@ struct DevBuf {
   operator T *() const { return p; }
 };
 
+template <typename T>
+struct PinnedBuf {                            // page-locked host memory (async device-to-host copies)
+  T *p = nullptr;
+  explicit PinnedBuf(ptn_handle h) {
+    HIPCHK(hipHostMalloc((void **)&p, sizeof(T), hipHostMallocDefault));
+    std::memset((void *)p, 0, sizeof(T));
+  }
+  ~PinnedBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+  PinnedBuf(const PinnedBuf &) = delete;
+  PinnedBuf &operator=(const PinnedBuf &) = delete;
+};
+
 ptn_status run(ptn_handle h, const std::function<void()> &fn);
 
 int round_up16(int R) { return (R + 15) / 16 * 16; }
@@ -117,11 +131,14 @@ struct Work {
   DevBuf<uint16_t> whi, wlo;
   DevBuf<NnlsParams> prm;
   DevBuf<uint64_t> mask;
+  DevBuf<int> flags;                         // [0] line-search step accepted, [1] threshold stop
   int64_t part_cap = 0, part32_cap = 0, wcap = 0, hist_cap = 0;
   struct NodeRef {
     const double *p64 = nullptr;
     const float *p32 = nullptr;
     int64_t ld = 0;
+    const int *sel = nullptr;
+    int64_t sel_off = 0;
   } left, right;                             // dimension-tree nodes cached between the modes of a sweep
   bool left_ready = false;                   // left node already formed for the coming mode 0 (line-search pass)
 
@@ -143,7 +160,7 @@ struct Work {
     grams.alloc(h, (size_t)N * R * R); agr.alloc(h, (size_t)N * R * R);
     Vm.alloc(h, (size_t)(N + 1) * R * R); Vacc.alloc(h, (size_t)R * R);
     M.alloc(h, maxI * ldr); Aold.alloc(h, maxI * ldr); colsum.alloc(h, kMaxRank);
-    prm.alloc(h, kMaxOrder + 1); mask.alloc(h, nnls_mask_words(maxI));
+    prm.alloc(h, kMaxOrder + 1); mask.alloc(h, nnls_mask_words(maxI)); flags.alloc(h, 4);
     nscr.alloc(h, 2 * kMaxOrder * kMaxRank); scal.alloc(h, 8); betas.alloc(h, (size_t)(N + 1) * (kMaxInner + 1));
     gpart.alloc(h, (size_t)((maxI + 31) / 32) * R * R);
     const int64_t C = total / dims[0];
@@ -200,11 +217,14 @@ struct Work {
   }
 };
 
-// A contracted node: rows of R values, FP64 (ld = ldr) or FP32 (ld = rpf).
+// A contracted node: rows of R values, FP64 (ld = ldr) or FP32 (ld = rpf).  With sel
+// (device int) the node starts sel_off values further when *sel == 1.
 struct Node {
   const double *p64;
   const float *p32;
   int64_t ld;
+  const int *sel = nullptr;
+  int64_t sel_off = 0;
 };
 
 int pick_splits(int64_t blocks_per_split, int64_t work_per_split_unit, int64_t max_split) {
@@ -287,10 +307,12 @@ void contract_node(Work &w, const Node &nd, const KrSpec &sp, int outmode, doubl
     const int64_t D0 = sp.dim[0], D1 = sp.dim[1];
     const int S = outmode == 0 ? pick_splits((D0 + 7) / 8, D1 / 16, 64) : pick_splits(D1, D0 / 64, 64);
     REQUIRE(S * In * w.ldr <= w.part_cap, PTN_ERR_INVALID, "partial buffer too small");
-    contract2(s, nd.p64, nd.p32, nd.ld, D0, D1, outmode, sp.fac[1 - outmode], w.ldr, w.R, S, w.part);
+    contract2(s, nd.p64, nd.p32, nd.ld, D0, D1, outmode, sp.fac[1 - outmode], w.ldr, w.R, S, w.part, nd.sel,
+              nd.sel_off);
     reduce_splits(s, w.part, S, In * w.ldr, M);
     return;
   }
+  REQUIRE(nd.sel == nullptr, PTN_ERR_INVALID, "device-selected node needs two digits");
   const int S = pick_splits(In, Cother / 64, 64);
   REQUIRE(S * In * w.ldr <= w.part_cap, PTN_ERR_INVALID, "partial buffer too small");
   contract_rows(s, nd.p64, nd.p32, nd.ld, sp, outmode, w.ldr, S, w.part);
@@ -336,8 +358,10 @@ void mttkrp_standard(Work &w, double *base, int mode, double *M) {
 void mttkrp_tree(Work &w, double *base, int mode, double *M) {
   const int N = w.N;
   const int64_t *d = w.dims;
-  auto keep = [](Work::NodeRef &dst, const Node &n) { dst.p64 = n.p64; dst.p32 = n.p32; dst.ld = n.ld; };
-  auto node = [](const Work::NodeRef &r) { return Node{r.p64, r.p32, r.ld}; };
+  auto keep = [](Work::NodeRef &dst, const Node &n) {
+    dst.p64 = n.p64; dst.p32 = n.p32; dst.ld = n.ld; dst.sel = nullptr; dst.sel_off = 0;
+  };
+  auto node = [](const Work::NodeRef &r) { return Node{r.p64, r.p32, r.ld, r.sel, r.sel_off}; };
   if (N == 3) {
     if (mode == 0 && !w.left_ready)
       keep(w.left, node_cm(w, kr_of(w, base, {2}), d[0] * d[1], d[2], 1, w.YL.p, w.YL32.p));
@@ -754,17 +778,23 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
       HIPCHK(hipStreamWaitEvent(ms, h->ev[1], 0));
     };
     ModeIn sel[kMaxOrder];
-    int sel0 = 0;                    // slot of mode 0's set-up, 0 or N (prepared during the line search)
     bool prepped0 = false;
+    // per-sweep read-back (pinned, filled by async copies in stream order): one host
+    // synchronisation per sweep; the line-search decision is taken on the device
+    struct Readback {
+      NnlsParams hp[kMaxOrder + 1];
+      double f, fa;
+      int flags[4];
+    };
+    PinnedBuf<Readback> rbuf(h);
+    Readback *rb = rbuf.p;
     auto t0 = std::chrono::steady_clock::now();
-    NnlsParams hp[kMaxOrder + 1];
     for (;;) {
       r.ao_iter++;
       ptn_sweep_log lg{};
       for (int n = 0; n < N; ++n) {
         if (!(n == 0 && prepped0)) {
           sel[n] = slot(w, n);
-          if (n == 0) sel0 = 0;
           fork();
           prep_mode(w, w.grams, n, tree, sel[n], o.max_inner, ss);
         }
@@ -780,27 +810,21 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
       if (o.normalization && !all_orth) {
         double *gp[kMaxOrder], *fp[kMaxOrder];
         for (int k = 0; k < N; ++k) { gp[k] = w.G(k); fp[k] = w.F(k); }
-        normalize(h->stream, N, w.dims, R, w.ldr, wf, gp, fp, w.nscr);
+        normalize(ms, N, w.dims, R, w.ldr, wf, gp, fp, w.nscr);
       }
-      HIPCHK(hipMemcpyAsync(hp, w.prm.p, sizeof(NnlsParams) * (N + 1), hipMemcpyDeviceToHost, h->stream));
-      r.f_value = read_scalar(h, w.scal.p + 1);
-      r.rel_cost = r.f_value / std::sqrt(r.frob_sq);
-      lg.f_value = r.f_value;
-      lg.rel_cost = r.rel_cost;
-      for (int n = 0; n < N; ++n) {
-        const NnlsParams &q = hp[n == 0 ? sel0 : n];
-        lg.nnls_iters[n] = q.iters;
-        lg.zero_col_fallback[n] = q.fallback;
-        lg.lipschitz[n] = q.L0;
-        lg.mu[n] = q.mu0;
-      }
-      const bool stop = r.rel_cost < o.threshold_error || r.ao_iter >= o.max_iter;
-      if (!stop && o.acceleration) {
-        HIPCHK(hipMemcpyAsync(w.nrm, w.fac, w.fsz * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+      HIPCHK(hipMemcpyAsync(rb->hp, w.prm.p, sizeof(NnlsParams) * (N + 1), hipMemcpyDeviceToHost, ms));
+      HIPCHK(hipMemcpyAsync(&rb->f, w.scal.p + 1, sizeof(double), hipMemcpyDeviceToHost, ms));
+      // termination (Cpd.hpp:339-344): the iteration cap is known here; the threshold
+      // test is repeated on the device, which then accepts no step
+      const bool cap = r.ao_iter >= o.max_iter;
+      bool tried = false;
+      if (!cap && o.acceleration) {
+        HIPCHK(hipMemcpyAsync(w.nrm, w.fac, w.fsz * sizeof(double), hipMemcpyDeviceToDevice, ms));
         if (r.ao_iter > 1) {
           // line_search_accel, Cpd.hpp:247-278
+          tried = true;
           double step = std::pow((double)(r.ao_iter + 1), 1.0 / (double)r.accel_coeff);
-          axpy_accel(h->stream, w.fsz, step, w.old, w.fac, w.acc);
+          axpy_accel(ms, w.fsz, step, w.old, w.fac, w.acc);
           for (int k = 0; k < N; ++k) gram_of(w, w.Fb(w.acc, k), w.dims[k], w.agr.p + (int64_t)k * R * R);
           // accel_cost_function (CpdDimTree.hpp:174-181): its fit term from the last mode's
           // MTTKRP, or from mode 0 of the paired left-node pass
@@ -821,37 +845,59 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
           hadamard(ms, R, nl, gp, w.Vacc);
           cost_gram(ms, w.dims[cm] * w.ldr, w.M, w.Fb(w.acc, cm), R, w.Vacc, w.agr.p + (int64_t)cm * R * R,
                     w.scal.p, w.scal.p + 2, 1);
-          if (pair) join();          // the side stream has read the Grams before they change
-          double fa = read_scalar(h, w.scal.p + 2);
-          lg.accel_tried = 1;
-          lg.f_accel = fa;
-          if (r.f_value > fa) {
-            HIPCHK(hipMemcpyAsync(w.fac, w.acc, w.fsz * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
-            HIPCHK(hipMemcpyAsync(w.grams, w.agr, (size_t)N * R * R * sizeof(double), hipMemcpyDeviceToDevice,
-                                  h->stream));
-            r.f_value = fa;
-            lg.accel_accepted = 1;
-          } else {
-            r.accel_fail++;
+          if (pair) join();          // the side stream has finished with the Grams and slots
+          AccelCopy cp{};
+          cp.src[0] = w.acc; cp.dst[0] = w.fac; cp.n[0] = w.fsz;
+          cp.src[1] = w.agr; cp.dst[1] = w.grams; cp.n[1] = (int64_t)N * R * R;
+          if (pair) {                // the accelerated candidate's mode-0 set-up into slot 0
+            const ModeIn a = slot(w, N), b = slot(w, 0);
+            cp.src[2] = a.V; cp.dst[2] = b.V; cp.n[2] = (int64_t)R * R;
+            cp.src[3] = a.betas; cp.dst[3] = b.betas; cp.n[3] = kMaxInner + 1;
+            cp.prm_src = a.p; cp.prm_dst = b.p;
           }
+          accel_decide(ms, w.scal.p, o.threshold_error, w.flags, cp);
+          HIPCHK(hipMemcpyAsync(&rb->fa, w.scal.p + 2, sizeof(double), hipMemcpyDeviceToHost, ms));
+          HIPCHK(hipMemcpyAsync(rb->flags, w.flags.p, 2 * sizeof(int), hipMemcpyDeviceToHost, ms));
           if (pair) {
-            // the next sweep's left node and mode-0 set-up: those that match the kept factors
+            // the next sweep's left node: the half of the pair the device kept
             w.left.p64 = nullptr;
-            w.left.p32 = w.YL2.p + (lg.accel_accepted ? w.rpf : 0);
+            w.left.p32 = w.YL2.p;
             w.left.ld = 2 * w.rpf;
+            w.left.sel = w.flags.p;
+            w.left.sel_off = w.rpf;
             w.left_ready = true;
-            sel0 = lg.accel_accepted ? N : 0;
-            sel[0] = slot(w, sel0);
+            sel[0] = slot(w, 0);
             prepped0 = true;
           }
-          if (r.accel_fail == 5) {
-            r.accel_fail = 0;
-            r.accel_coeff++;
-          }
         }
-        HIPCHK(hipMemcpyAsync(w.old, w.nrm, w.fsz * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(w.old, w.nrm, w.fsz * sizeof(double), hipMemcpyDeviceToDevice, ms));
       }
       HIPCHK(hipStreamSynchronize(ms));
+      r.f_value = rb->f;
+      r.rel_cost = r.f_value / std::sqrt(r.frob_sq);
+      lg.f_value = r.f_value;
+      lg.rel_cost = r.rel_cost;
+      for (int n = 0; n < N; ++n) {
+        lg.nnls_iters[n] = rb->hp[n].iters;
+        lg.zero_col_fallback[n] = rb->hp[n].fallback;
+        lg.lipschitz[n] = rb->hp[n].L0;
+        lg.mu[n] = rb->hp[n].mu0;
+      }
+      const bool stop = r.rel_cost < o.threshold_error || cap;
+      if (tried && !stop) {
+        lg.accel_tried = 1;
+        lg.f_accel = rb->fa;
+        if (rb->flags[0]) {
+          r.f_value = rb->fa;
+          lg.accel_accepted = 1;
+        } else {
+          r.accel_fail++;
+        }
+        if (r.accel_fail == 5) {
+          r.accel_fail = 0;
+          r.accel_coeff++;
+        }
+      }
       lg.t_end = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
       if (log && r.ao_iter - 1 < log_cap) log[r.ao_iter - 1] = lg;
       if (stop) break;

@@ -67,8 +67,10 @@ void contract_rows(hipStream_t s, const double *T64, const float *T32, int64_t l
 void reduce_splits(hipStream_t s, const double *partials, int splits, int64_t n, double *out);
 // Two-digit node Y (rows i0 + D0 i1, ld values per row) contracted against the other
 // digit's factor F: part[split][i][ldr] for the kept digit i (outmode 0: i0, 1: i1).
+// sel (optional, device): the node starts at Y + (*sel) * sel_off.
 void contract2(hipStream_t s, const double *Y64, const float *Y32, int64_t ld, int64_t D0, int64_t D1, int outmode,
-               const double *F, int ldr, int R, int splits, double *part);
+               const double *F, int ldr, int R, int splits, double *part, const int *sel = nullptr,
+               int64_t sel_off = 0);
 
 // FP32-stream MFMA variants (bf16x3 split products or FP32 MFMA), FP32 outputs with
 // RPF = R rounded up to 32 columns.  The Khatri-Rao operand is packed once per
@@ -114,6 +116,16 @@ void cost_gram(hipStream_t s, int64_t nma, const double *M, const double *A, int
                const double *frob, double *f, int accel);
 void normalize(hipStream_t s, int N, const int64_t *dims, int R, int ldr, int wf, double *const *grams,
                double *const *fac, double *scratch);
+// Conditional copies of an accepted line-search step: four double ranges and one
+// NnlsParams (any may be empty / null).
+struct AccelCopy {
+  const double *src[4];
+  double *dst[4];
+  int64_t n[4];
+  const NnlsParams *prm_src;
+  NnlsParams *prm_dst;
+};
+void accel_decide(hipStream_t s, const double *scal, double thr, int *flags, const AccelCopy &cp);
 void axpy_accel(hipStream_t s, int64_t n, double step, const double *old, const double *cur, double *out);
 
 }  // namespace ptn

@@ -590,6 +590,27 @@ __global__ void k_axpy_accel(int64_t n, double step, const double *old, const do
     out[e] = old[e] + step * (cur[e] - old[e]);
 }
 
+// Line-search decision on the device (Cpd.hpp:262-269): accept the accelerated factors
+// when f > f_hat, unless the run stops on the threshold at this sweep (the reference
+// checks termination first, Cpd.hpp:339-346).  On acceptance the accelerated factors,
+// their Grams and the mode-0 NNLS set-up prepared for them replace the current ones.
+// flags[0] = accepted, flags[1] = threshold stop.
+__global__ void k_accel_decide(const double *__restrict__ scal, double thr, int *flags, AccelCopy cp) {
+  const double f = scal[1], fa = scal[2];
+  const bool stop = f / sqrt(scal[0]) < thr;
+  const bool accept = !stop && f > fa;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    flags[0] = accept ? 1 : 0;
+    flags[1] = stop ? 1 : 0;
+  }
+  if (!accept) return;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int q = 0; q < 4; ++q)
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < cp.n[q]; e += stride)
+      cp.dst[q][e] = cp.src[q][e];
+  if (cp.prm_src && blockIdx.x == 0 && threadIdx.x == 0) *cp.prm_dst = *cp.prm_src;
+}
+
 int grid_for(int64_t n, int block) {
   int64_t g = (n + block - 1) / block;
   if (g > 65536) g = 65536;
@@ -671,6 +692,10 @@ void normalize(hipStream_t s, int N, const int64_t *dims, int R, int ldr, int wf
   for (int k = 0; k < N; ++k) maxI = dims[k] > maxI ? dims[k] : maxI;
   k_normalize<<<1, 256, 0, s>>>(na, scratch);
   k_normalize_rows<<<dim3((unsigned)((maxI + 3) / 4), (unsigned)N), 256, 0, s>>>(na, scratch);
+}
+
+void accel_decide(hipStream_t s, const double *scal, double thr, int *flags, const AccelCopy &cp) {
+  k_accel_decide<<<grid_for(cp.n[0] > 4096 ? cp.n[0] : 4096, 256), 256, 0, s>>>(scal, thr, flags, cp);
 }
 
 void axpy_accel(hipStream_t s, int64_t n, double step, const double *old, const double *cur, double *out) {

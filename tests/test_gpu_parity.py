@@ -198,6 +198,24 @@ def test_als_c1_parity(h, form):
     _compare_runs(Fg, stg, logg, Fo, sto, logo, 1e-10)
 
 
+@pytest.mark.parametrize("form", [P.MTTKRP_STANDARD, P.MTTKRP_DIMTREE])
+def test_als_threshold_stop(h, form):
+    """The relative-cost threshold ends the run at the same sweep as the oracle; the
+    stopping sweep takes no acceleration step (termination precedes line_search_accel)."""
+    dims, R = [60, 50, 40], 6
+    X, _ = o.make_tensor(dims, R, seed=8, snr_db=30.0)
+    init = o.gen_factors(dims, R, seed=9)
+    h.set_tensor(X)
+    kw = dict(max_iter=200, threshold_error=0.05)
+    Fg, stg, logg = h.cpd(R, init, mttkrp_form=form, **kw)
+    Fo, sto, logo = o.cpd(X, R, init, mttkrp_form=(o.MTTKRP_DIMTREE if form == P.MTTKRP_DIMTREE else
+                                                     o.MTTKRP_STANDARD), cost_form=o.COST_GRAM, **kw)
+    say("threshold stop at", stg["ao_iter"], "oracle", sto["ao_iter"])
+    assert 1 < sto["ao_iter"] < 200
+    assert logg[stg["ao_iter"] - 1]["accel_tried"] == 0
+    _compare_runs(Fg, stg, logg, Fo, sto, logo, 1e-10)
+
+
 def test_als_c1_no_accel_no_norm(h):
     say("ALS C1 variant: acceleration and normalization off")
     dims, R = [100, 100, 100], 10
