@@ -96,6 +96,19 @@ struct PinnedBuf {                            // page-locked host memory (async 
   PinnedBuf &operator=(const PinnedBuf &) = delete;
 };
 
+struct EventPair {                           // two timing-free events, destroyed with the scope
+  hipEvent_t e[2] = {nullptr, nullptr};
+  explicit EventPair(ptn_handle h) {
+    for (hipEvent_t &x : e) HIPCHK(hipEventCreateWithFlags(&x, hipEventDisableTiming));
+  }
+  ~EventPair() {
+    for (hipEvent_t x : e)
+      if (x) (void)hipEventDestroy(x);
+  }
+  EventPair(const EventPair &) = delete;
+  EventPair &operator=(const EventPair &) = delete;
+};
+
 ptn_status run(ptn_handle h, const std::function<void()> &fn);
 
 int round_up16(int R) { return (R + 15) / 16 * 16; }
@@ -779,19 +792,31 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
     };
     ModeIn sel[kMaxOrder];
     bool prepped0 = false;
-    // per-sweep read-back (pinned, filled by async copies in stream order): one host
-    // synchronisation per sweep; the line-search decision is taken on the device
+    // The host enqueues sweep it + 1 before it reads sweep it back (pinned buffers, one
+    // event per sweep), so the device never waits for the host.  The line-search
+    // decision and its counters live on the device (flags[0..3]); a sweep that ends the
+    // run on the threshold is found one sweep late, and the factors come from its
+    // snapshot.
     struct Readback {
       NnlsParams hp[kMaxOrder + 1];
       double f, fa;
       int flags[4];
     };
-    PinnedBuf<Readback> rbuf(h);
-    Readback *rb = rbuf.p;
-    auto t0 = std::chrono::steady_clock::now();
-    for (;;) {
-      r.ao_iter++;
-      ptn_sweep_log lg{};
+    PinnedBuf<Readback> rbuf0(h), rbuf1(h);
+    Readback *rbs[2] = {rbuf0.p, rbuf1.p};
+    DevBuf<double> snap0(h, w.fsz), snap1(h, w.fsz);
+    double *snaps[2] = {snap0.p, snap1.p};
+    EventPair evs(h);
+    {
+      const int fl[4] = {0, 0, o.accel_fail, o.accel_coeff};
+      HIPCHK(hipMemcpy(w.flags.p, fl, sizeof(fl), hipMemcpyHostToDevice));
+    }
+    int c_known = o.accel_coeff;     // coefficient after the last sweep read back
+    std::chrono::steady_clock::time_point t0;
+    bool tried_at[2] = {false, false};
+
+    auto enqueue = [&](int it) {
+      Readback *rb = rbs[it & 1];
       for (int n = 0; n < N; ++n) {
         if (!(n == 0 && prepped0)) {
           sel[n] = slot(w, n);
@@ -815,16 +840,18 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
       HIPCHK(hipMemcpyAsync(rb->hp, w.prm.p, sizeof(NnlsParams) * (N + 1), hipMemcpyDeviceToHost, ms));
       HIPCHK(hipMemcpyAsync(&rb->f, w.scal.p + 1, sizeof(double), hipMemcpyDeviceToHost, ms));
       // termination (Cpd.hpp:339-344): the iteration cap is known here; the threshold
-      // test is repeated on the device, which then accepts no step
-      const bool cap = r.ao_iter >= o.max_iter;
+      // test is repeated on the device, which then takes no step
+      const bool cap = it >= (int)o.max_iter;
       bool tried = false;
       if (!cap && o.acceleration) {
         HIPCHK(hipMemcpyAsync(w.nrm, w.fac, w.fsz * sizeof(double), hipMemcpyDeviceToDevice, ms));
-        if (r.ao_iter > 1) {
-          // line_search_accel, Cpd.hpp:247-278
+        if (it > 1) {
+          // line_search_accel, Cpd.hpp:247-278: step (k+1)^(1/accel_coeff); the device's
+          // coefficient is c_known or, after one more fifth failure, c_known + 1
           tried = true;
-          double step = std::pow((double)(r.ao_iter + 1), 1.0 / (double)r.accel_coeff);
-          axpy_accel(ms, w.fsz, step, w.old, w.fac, w.acc);
+          const double step_c = std::pow((double)(it + 1), 1.0 / (double)c_known);
+          const double step_c1 = std::pow((double)(it + 1), 1.0 / (double)(c_known + 1));
+          axpy_accel(ms, w.fsz, step_c, step_c1, c_known, w.flags.p + 3, w.old, w.fac, w.acc);
           for (int k = 0; k < N; ++k) gram_of(w, w.Fb(w.acc, k), w.dims[k], w.agr.p + (int64_t)k * R * R);
           // accel_cost_function (CpdDimTree.hpp:174-181): its fit term from the last mode's
           // MTTKRP, or from mode 0 of the paired left-node pass
@@ -857,7 +884,7 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
           }
           accel_decide(ms, w.scal.p, o.threshold_error, w.flags, cp);
           HIPCHK(hipMemcpyAsync(&rb->fa, w.scal.p + 2, sizeof(double), hipMemcpyDeviceToHost, ms));
-          HIPCHK(hipMemcpyAsync(rb->flags, w.flags.p, 2 * sizeof(int), hipMemcpyDeviceToHost, ms));
+          HIPCHK(hipMemcpyAsync(rb->flags, w.flags.p, 4 * sizeof(int), hipMemcpyDeviceToHost, ms));
           if (pair) {
             // the next sweep's left node: the half of the pair the device kept
             w.left.p64 = nullptr;
@@ -872,7 +899,16 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
         }
         HIPCHK(hipMemcpyAsync(w.old, w.nrm, w.fsz * sizeof(double), hipMemcpyDeviceToDevice, ms));
       }
-      HIPCHK(hipStreamSynchronize(ms));
+      HIPCHK(hipMemcpyAsync(snaps[it & 1], w.fac, w.fsz * sizeof(double), hipMemcpyDeviceToDevice, ms));
+      HIPCHK(hipEventRecord(evs.e[it & 1], ms));
+      tried_at[it & 1] = tried;
+    };
+
+    auto process = [&](int it) {
+      HIPCHK(hipEventSynchronize(evs.e[it & 1]));
+      const Readback *rb = rbs[it & 1];
+      ptn_sweep_log lg{};
+      r.ao_iter = it;
       r.f_value = rb->f;
       r.rel_cost = r.f_value / std::sqrt(r.frob_sq);
       lg.f_value = r.f_value;
@@ -883,29 +919,38 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
         lg.lipschitz[n] = rb->hp[n].L0;
         lg.mu[n] = rb->hp[n].mu0;
       }
-      const bool stop = r.rel_cost < o.threshold_error || cap;
-      if (tried && !stop) {
+      const bool stop = r.rel_cost < o.threshold_error || it >= (int)o.max_iter;
+      if (tried_at[it & 1] && !stop) {
         lg.accel_tried = 1;
         lg.f_accel = rb->fa;
         if (rb->flags[0]) {
           r.f_value = rb->fa;
           lg.accel_accepted = 1;
-        } else {
-          r.accel_fail++;
         }
-        if (r.accel_fail == 5) {
-          r.accel_fail = 0;
-          r.accel_coeff++;
-        }
+        r.accel_fail = rb->flags[2];
+        r.accel_coeff = rb->flags[3];
+        c_known = rb->flags[3];
       }
       lg.t_end = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-      if (log && r.ao_iter - 1 < log_cap) log[r.ao_iter - 1] = lg;
-      if (stop) break;
+      if (log && it - 1 < log_cap) log[it - 1] = lg;
+      return stop;
+    };
+
+    t0 = std::chrono::steady_clock::now();
+    int final_it = 1;
+    enqueue(1);
+    for (int it = 1;; ++it) {
+      if (it < (int)o.max_iter) enqueue(it + 1);
+      if (process(it)) {
+        final_it = it;
+        break;
+      }
     }
     HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipStreamSynchronize(h->side));
     r.sweep_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     HIPCHK(hipGetLastError());
-    download_factors(w, w.fac, out_packed);
+    download_factors(w, snaps[final_it & 1], out_packed);
     *res = r;
   });
 }

@@ -585,7 +585,12 @@ __global__ __launch_bounds__(256) void k_normalize_rows(NormArgs na, const doubl
   na.A[k][row * na.ldr + c] = na.A[k][row * na.ldr + c] * sc;
 }
 
-__global__ void k_axpy_accel(int64_t n, double step, const double *old, const double *cur, double *out) {
+// A_hat = A_old + step (A - A_old) (Cpd.hpp:251-258).  The step depends on the current
+// acceleration coefficient, which the device updates: step_c for *coeff == c, else
+// step_c1 (the coefficient rises by at most one per sweep).
+__global__ void k_axpy_accel(int64_t n, double step_c, double step_c1, int c, const int *coeff, const double *old,
+                             const double *cur, double *out) {
+  const double step = (coeff == nullptr || *coeff == c) ? step_c : step_c1;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
     out[e] = old[e] + step * (cur[e] - old[e]);
 }
@@ -594,7 +599,8 @@ __global__ void k_axpy_accel(int64_t n, double step, const double *old, const do
 // when f > f_hat, unless the run stops on the threshold at this sweep (the reference
 // checks termination first, Cpd.hpp:339-346).  On acceptance the accelerated factors,
 // their Grams and the mode-0 NNLS set-up prepared for them replace the current ones.
-// flags[0] = accepted, flags[1] = threshold stop.
+// flags[0] = accepted, flags[1] = threshold stop, flags[2] / flags[3] = accel_fail /
+// accel_coeff, kept as line_search_accel keeps them (Cpd.hpp:270-277).
 __global__ void k_accel_decide(const double *__restrict__ scal, double thr, int *flags, AccelCopy cp) {
   const double f = scal[1], fa = scal[2];
   const bool stop = f / sqrt(scal[0]) < thr;
@@ -602,6 +608,14 @@ __global__ void k_accel_decide(const double *__restrict__ scal, double thr, int 
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     flags[0] = accept ? 1 : 0;
     flags[1] = stop ? 1 : 0;
+    if (!stop) {
+      int fail = flags[2] + (accept ? 0 : 1);
+      if (fail == 5) {
+        fail = 0;
+        flags[3] = flags[3] + 1;
+      }
+      flags[2] = fail;
+    }
   }
   if (!accept) return;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -698,8 +712,9 @@ void accel_decide(hipStream_t s, const double *scal, double thr, int *flags, con
   k_accel_decide<<<grid_for(cp.n[0] > 4096 ? cp.n[0] : 4096, 256), 256, 0, s>>>(scal, thr, flags, cp);
 }
 
-void axpy_accel(hipStream_t s, int64_t n, double step, const double *old, const double *cur, double *out) {
-  k_axpy_accel<<<grid_for(n, 256), 256, 0, s>>>(n, step, old, cur, out);
+void axpy_accel(hipStream_t s, int64_t n, double step_c, double step_c1, int c, const int *coeff, const double *old,
+                const double *cur, double *out) {
+  k_axpy_accel<<<grid_for(n, 256), 256, 0, s>>>(n, step_c, step_c1, c, coeff, old, cur, out);
 }
 
 }  // namespace ptn
