@@ -102,6 +102,48 @@ def copy_calibration(nbytes):
     log("copy calibration: 3 device copies of %d bytes" % nbytes)
 
 
+def bench_gemm_node():
+    """The order-4 dimension tree's first contraction, X_(0,1) (A_3 (.) A_2), on a dense
+    200^4 FP32 tensor (true rank 64, Exponential(1) factor columns from seed 2, whose
+    expected pairwise cosine is 1/2, SNR 20 dB), R = 64: FP32 MFMA vs bf16x3 split
+    products (FP32 accumulate), with the FP64 MFMA node on the FP64 copy as the
+    reference; errors are those of the tree's four MTTKRPs against FP64."""
+    dims, Rn = [200, 200, 200, 200], 64
+    h = P.Handle(0)
+    t = time.time()
+    h.generate_tensor(dims, Rn, seed=2, snr_db=20.0, dist=1, storage=P.TENSOR_BOTH)
+    A = h.generate_factors(dims, Rn, seed=1)
+    log("200^4 tensor generated in %.2f s" % (time.time() - t))
+    M64 = h.mttkrp_dimtree(A, arith=P.ARITH_F64)
+    flops = 2.0 * Rn * float(np.prod(dims))
+    res = {}
+    for name, ar in (("fp32_mfma", P.ARITH_F32), ("bf16x3", P.ARITH_BF16X3), ("fp64_mfma", P.ARITH_F64)):
+        ms, b = h.time_contraction(0, A, arith=ar, iters=10)
+        entry = {"ms": ms, "gbs": b / (ms * 1e-3) / 1e9, "tflops": flops / (ms * 1e-3) / 1e12,
+                 "algorithmic_bytes": b}
+        if ar != P.ARITH_F64:
+            M = h.mttkrp_dimtree(A, arith=ar)
+            entry["mttkrp_rel_err_vs_fp64"] = [P.rel_err(M[k], M64[k]) for k in range(4)]
+        res[name] = entry
+        log("%s: %.3f ms, %.0f GB/s, %.1f TFLOP/s" % (name, ms, entry["gbs"], entry["tflops"]))
+    h.close()
+    best = min(("fp32_mfma", "bf16x3"), key=lambda k: res[k]["ms"])
+    out = {
+        "metric": "order-4 GEMM node X_(0,1)(A3 (.) A2) time, 200^4 FP32, R=64: FP32 MFMA vs bf16x3",
+        "value": res[best]["ms"], "unit": "ms", "higher_is_better": False, "n_gpus": 1, "dtype": "f32",
+        "data": "synthetic (device Philox4x32-10: Exponential(1) factors seed 2, noise at SNR 20 dB; "
+                "evaluation factors U[0,1) seed 1)",
+        "config": {"workload": "dense 4-way 200x200x200x200, true rank 64, R=64, fused GEMM+TTV node "
+                               "(no intermediate), split over k and reduced", "fastest": best, **res},
+        "roofline": {"fp32_mfma": {"bound": "mfma", "peak": 157.3, "unit": "TFLOP/s",
+                                   "achieved": res["fp32_mfma"]["tflops"],
+                                   "frac": res["fp32_mfma"]["tflops"] / 157.3},
+                     "bf16x3": {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                "achieved": res["bf16x3"]["gbs"], "frac": res["bf16x3"]["gbs"] / HBM_PEAK_GBS}},
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,7 +154,12 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 GPU run")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (HBM counter calibration) and exit")
+    ap.add_argument("--gemm-node", action="store_true",
+                    help="only time the order-4 GEMM node (200^4, R=64, FP32 MFMA vs bf16x3) and exit")
     args = ap.parse_args()
+    if args.gemm_node:
+        bench_gemm_node()
+        return
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return

@@ -131,7 +131,8 @@ ptn_status ptn_cost_explicit(ptn_handle h, int R, const double *factors_packed, 
 
 /* Average device time (ms) of `iters` back-to-back launches of one contraction
    kernel on the handle's stream, bracketed by HIP events on that stream.
-   kernel: 0 = left node X x_{N-1} A_{N-1} (gemm_cm), 1 = right node X x_0 A_0 (gemm_ck; for an
+   kernel: 0 = left node X x_{N-1} A_{N-1} (gemm_cm; for an order-4 tensor the fused node
+   X_(0,1) (A_3 (.) A_2) with its split-k reduction), 1 = right node X x_0 A_0 (gemm_ck; for an
    order-3 tensor on the FP32-output paths fused with the TTV by A_1, i.e. the mode-2 MTTKRP),
    2 = mode-0 standard MTTKRP.  Operand packing is done before the timed launches.
    bytes_out: algorithmic bytes of one launch (tensor once + output + factor rows). */

@@ -1106,19 +1106,32 @@ ptn_status ptn_time_contraction(ptn_handle h, int kernel, int R, const double *f
     check_arith(h, arith);
     REQUIRE(R >= 1 && R <= PTN_MAX_RANK, PTN_ERR_UNSUPPORTED, "rank must be in [1, 64]");
     REQUIRE(kernel >= 0 && kernel <= 2 && iters >= 1 && avg_ms, PTN_ERR_INVALID, "bad argument");
-    REQUIRE(h->N == 3 || kernel == 2, PTN_ERR_UNSUPPORTED, "node timing built for N = 3");
+    REQUIRE(h->N == 3 || kernel == 2 || (kernel == 0 && h->N == 4), PTN_ERR_UNSUPPORTED,
+            "node timing built for N = 3 (and the order-4 left node)");
     Work w(h, R, arith, true);
     upload_factors(w, factors_packed, w.fac);
     const int64_t *d = w.dims;
     const double sx = arith == PTN_ARITH_F64 ? 8.0 : 4.0;
     // operands packed once; the timed launches are the contraction kernel alone
     const bool fused = kernel == 1 && w.mfma32 && w.N == 3;
+    const bool n4 = w.N == 4;
+    // order 4: the fused GEMM + TTV node X_(0,1) (A_3 (.) A_2), split over k and reduced
+    const int S4 = n4 ? pick_splits((d[0] * d[1] + 255) / 256, d[2] * d[3] / 256, 64) : 1;
     if (w.mfma32) {
-      if (kernel == 0) w.pack(kr_of(w, w.fac, {2}), d[2], 0);
+      if (kernel == 0) w.pack(n4 ? kr_of(w, w.fac, {2, 3}) : kr_of(w, w.fac, {2}), n4 ? d[2] * d[3] : d[2], 0);
       else if (kernel == 1) w.pack(kr_of(w, w.fac, {0}), d[0], 1);
     }
     auto launch = [&] {
-      if (kernel == 0) {
+      if (kernel == 0 && n4) {
+        const int64_t Mr = d[0] * d[1], K = d[2] * d[3];
+        if (w.mfma32) {
+          REQUIRE(S4 * Mr * w.rpf <= w.part32_cap, PTN_ERR_INVALID, "partial buffer too small");
+          gemm_cm_f32(h->stream, w.gmode(), w.x32(), Mr, K, w.rpf, S4, w.whi, w.wlo, w.wf, w.part32);
+          reduce_splits_f32(h->stream, w.part32, S4, Mr * w.rpf, w.YL32);
+        } else {
+          node_cm(w, kr_of(w, w.fac, {2, 3}), Mr, K, S4, w.YL.p, w.YL32.p);
+        }
+      } else if (kernel == 0) {
         if (w.mfma32) gemm_cm_f32(h->stream, w.gmode(), w.x32(), d[0] * d[1], d[2], w.rpf, 1, w.whi, w.wlo, w.wf,
                                   w.YL32);
         else node_cm(w, kr_of(w, w.fac, {2}), d[0] * d[1], d[2], 1, w.YL.p, w.YL32.p);
@@ -1151,7 +1164,8 @@ ptn_status ptn_time_contraction(ptn_handle h, int kernel, int R, const double *f
       // algorithmic bytes: tensor once + output + factor reads
       double b = sx * (double)w.total;
       const double sy = w.mfma32 ? 4.0 : 8.0;   // node element size
-      if (kernel == 0) b += sy * R * (double)(d[0] * d[1]) + 8.0 * R * d[2];
+      if (kernel == 0 && n4) b += sy * R * (double)(d[0] * d[1]) + 8.0 * R * (double)(d[2] + d[3]);
+      else if (kernel == 0) b += sy * R * (double)(d[0] * d[1]) + 8.0 * R * d[2];
       else if (fused) b += 8.0 * R * d[2] + 8.0 * R * (double)(d[0] + d[1]);
       else if (kernel == 1) b += sy * R * (double)(w.total / d[0]) + 8.0 * R * d[0];
       else b += 8.0 * R * d[0] + 8.0 * R * (double)(d[1] + d[2]);

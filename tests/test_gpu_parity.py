@@ -169,6 +169,24 @@ def test_normalize_matches_oracle(h):
             assert np.array_equal(a, b)
 
 
+def test_order4_gemm_node_full_size_errors(h):
+    """200^4 FP32 (6.4 GB), R = 64, Exponential(1) factors: the tree's MTTKRPs from the
+    FP32-MFMA and bf16x3 nodes against the FP64-MFMA path on the FP64 copy (itself
+    pinned to the oracle at small sizes).  Tolerances: FP32 accumulation over k =
+    40000 in split chunks."""
+    dims, R = [200, 200, 200, 200], 64
+    say("order-4 node 200^4 R=64")
+    h.generate_tensor(dims, R, seed=2, snr_db=20.0, dist=1, storage=P.TENSOR_BOTH)
+    A = h.generate_factors(dims, R, seed=1)
+    M64 = h.mttkrp_dimtree(A, arith=P.ARITH_F64)
+    for ar, tol in ((P.ARITH_F32, 1e-5), (P.ARITH_BF16X3, 1e-4)):
+        M = h.mttkrp_dimtree(A, arith=ar)
+        errs = [P.rel_err(M[k], M64[k]) for k in range(4)]
+        say("arith", ar, "rel errs", errs)
+        assert max(errs) < tol
+    h.set_tensor(np.ones((2, 2, 2), order="F"))          # release the 19 GB
+
+
 # --------------------------------------------------------------------- ALS --
 
 def _compare_runs(Fg, stg, logg, Fo, sto, logo, tol):
