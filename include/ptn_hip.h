@@ -139,6 +139,53 @@ ptn_status ptn_cost_explicit(ptn_handle h, int R, const double *factors_packed, 
 ptn_status ptn_time_contraction(ptn_handle h, int kernel, int R, const double *factors_packed, int arith, int iters,
                                 double *avg_ms, double *bytes_out);
 
+/* ---- batched independent CPDs.  No counterpart in the reference, which runs one cpd()
+        per tensor: many small order-3 non-negative problems of uneven sizes (modes in
+        [1, 128]) and ranks ([1, 16]) advanced together, one persistent kernel per ALS
+        sweep (the sweep of Cpd.hpp:288-356: standard MTTKRP, NesterovMNLS per factor,
+        Gram-form cost, Normalize, line_search_accel), tensors in FP32, arithmetic in
+        FP64 ---- */
+#define PTN_BATCH_MAX_DIM 128
+#define PTN_BATCH_MAX_RANK 16
+
+/* One record per problem and sweep. */
+typedef struct {
+  double f_value;      /* cost after the sweep's last factor update (before acceleration) */
+  double f_accel;      /* cost at the extrapolated point, 0 when not tried */
+  int    nnls_iters[3];
+  int    accel_tried, accel_accepted;
+  int    stopped;      /* the threshold test held after this sweep */
+} ptn_batch_log;
+
+typedef struct {
+  int    block, grid;  /* launch shape of the sweep kernel */
+  int    tuned;        /* 1 when the autotuner chose it */
+  int    sweeps;       /* sweeps run (every problem stops on its own threshold test before) */
+  double seconds;      /* device time of all sweeps (HIP events around the graph launches) */
+  double tune_seconds; /* wall time of the autotuner (not part of seconds) */
+} ptn_batch_report;
+
+/* The batch: dims [nprob][3], ranks [nprob]; X = the tensors back to back, each
+   column-major FP64 (stored as FP32 on the device).  Replaces any earlier batch. */
+ptn_status ptn_batch_set(ptn_handle h, int nprob, const int *dims, const int *ranks, const double *X);
+/* Synthetic batch on the device: problem p = [[A_p]] (rank ranks[p], U[0,1) factors) plus
+   Gaussian noise at snr_db (none when NaN), both drawn from seed + p with the generator
+   of ptn_generate_tensor, then rounded to FP32. */
+ptn_status ptn_batch_generate(ptn_handle h, int nprob, const int *dims, const int *ranks, uint64_t seed,
+                              double snr_db);
+/* U[0,1) initial factors, problem p from seed + p; packed per problem, problems back to back. */
+ptn_status ptn_batch_init_factors(ptn_handle h, uint64_t seed, double *packed_out);
+/* Tensor p of the batch as stored (FP32 values widened to FP64), column-major. */
+ptn_status ptn_batch_get_tensor(ptn_handle h, int p, double *X_out);
+/* ALS on every problem of the batch with the options of ptn_cpd (mttkrp_form, arith and
+   fuse_accel are not consulted).  init_packed / out_packed: factors packed per problem,
+   problems back to back; res: nprob records; log: NULL or [nprob][opt->max_iter].
+   block / grid: launch shape of the sweep kernel (256 or 512 threads), or both 0 to let
+   the autotuner time a few shapes on the first two sweeps (state restored afterwards).
+   rep: NULL or the shape used and the times. */
+ptn_status ptn_cpd_batch(ptn_handle h, const double *init_packed, const ptn_options *opt, double *out_packed,
+                         ptn_result *res, ptn_batch_log *log, int block, int grid, ptn_batch_report *rep);
+
 #ifdef __cplusplus
 }
 #endif

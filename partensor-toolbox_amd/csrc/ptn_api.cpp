@@ -12,11 +12,25 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
-#include <string>
+#include <memory>
 #include <new>
+#include <numeric>
+#include <string>
 #include <vector>
 
 using namespace ptn;
+
+// The tensors of a batch of small problems (ptn_batch_set / ptn_batch_generate): FP32,
+// back to back, each starting on a 64-element boundary.
+struct BatchSet {
+  int nprob = 0;
+  std::vector<BatchDesc> desc;   // host copy
+  int64_t nx = 0, nrows = 0;     // tensor elements (padded), factor rows over all problems
+  float *x32 = nullptr;
+  ~BatchSet() {
+    if (x32) (void)hipFree(x32);
+  }
+};
 
 struct ptn_handle_s {
   int device = 0;
@@ -33,6 +47,7 @@ struct ptn_handle_s {
   double *part = nullptr;    // reduction partials for norms
   bool frob_valid = false;
   double frob_host = 0.0;
+  BatchSet *batch = nullptr;
 };
 
 namespace {
@@ -604,6 +619,7 @@ void ptn_destroy(ptn_handle h) {
   if (h->side) (void)hipStreamSynchronize(h->side);
   if (h->x64) (void)hipFree(h->x64);
   if (h->x32) (void)hipFree(h->x32);
+  delete h->batch;
   if (h->scal) (void)hipFree(h->scal);
   if (h->part) (void)hipFree(h->part);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -1170,6 +1186,365 @@ ptn_status ptn_time_contraction(ptn_handle h, int kernel, int R, const double *f
       else if (kernel == 1) b += sy * R * (double)(w.total / d[0]) + 8.0 * R * d[0];
       else b += 8.0 * R * d[0] + 8.0 * R * (double)(d[1] + d[2]);
       *bytes_out = b;
+    }
+  });
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------
+// batched independent order-3 problems
+// ------------------------------------------------------------------------------------
+namespace {
+
+constexpr int kBatchBR = 16;        // factor rows of a batched problem hold 16 doubles
+constexpr int kBatchMaxDim = 128;
+constexpr int kBatchMaxRank = 16;
+
+static_assert(sizeof(BatchLog) == sizeof(ptn_batch_log), "ptn_batch_log layout");
+
+void batch_layout(ptn_handle h, int nprob, const int *dims, const int *ranks) {
+  REQUIRE(nprob >= 1 && nprob <= (1 << 20), PTN_ERR_INVALID, "nprob must be in [1, 2^20]");
+  REQUIRE(dims && ranks, PTN_ERR_INVALID, "dims or ranks is null");
+  std::unique_ptr<BatchSet> b(new BatchSet());
+  b->nprob = nprob;
+  b->desc.resize(nprob);
+  int64_t xo = 0, fo = 0;
+  for (int p = 0; p < nprob; ++p) {
+    BatchDesc &d = b->desc[p];
+    std::memset(&d, 0, sizeof(d));
+    for (int k = 0; k < 3; ++k) {
+      REQUIRE(dims[3 * p + k] >= 1 && dims[3 * p + k] <= kBatchMaxDim, PTN_ERR_UNSUPPORTED,
+              "batched problems take mode sizes in [1, 128]");
+      d.I[k] = dims[3 * p + k];
+    }
+    REQUIRE(ranks[p] >= 1 && ranks[p] <= kBatchMaxRank, PTN_ERR_UNSUPPORTED, "batched problems take ranks in [1, 16]");
+    d.R = ranks[p];
+    d.rowoff[0] = 0;
+    d.rowoff[1] = d.I[0];
+    d.rowoff[2] = d.I[0] + d.I[1];
+    d.x_off = xo;
+    xo += ((int64_t)d.I[0] * d.I[1] * d.I[2] + 63) / 64 * 64;
+    d.f_off = fo;
+    fo += d.I[0] + d.I[1] + d.I[2];
+    d.g_off = (int64_t)p * 3 * kBatchBR * kBatchBR;
+  }
+  b->nx = xo;
+  b->nrows = fo;
+  HIPCHK(hipMalloc((void **)&b->x32, xo * sizeof(float)));
+  HIPCHK(hipMemsetAsync(b->x32, 0, xo * sizeof(float), h->stream));
+  delete h->batch;
+  h->batch = b.release();
+}
+
+int64_t batch_elems(const BatchDesc &d) { return (int64_t)d.I[0] * d.I[1] * d.I[2]; }
+
+struct GraphExec {                       // a captured and instantiated graph, destroyed with the scope
+  hipGraph_t g = nullptr;
+  hipGraphExec_t e = nullptr;
+  ~GraphExec() {
+    if (e) (void)hipGraphExecDestroy(e);
+    if (g) (void)hipGraphDestroy(g);
+  }
+};
+
+struct TimingEvents {
+  hipEvent_t e[2] = {nullptr, nullptr};
+  explicit TimingEvents(ptn_handle h) {
+    for (hipEvent_t &x : e) HIPCHK(hipEventCreate(&x));
+  }
+  ~TimingEvents() {
+    for (hipEvent_t x : e)
+      if (x) (void)hipEventDestroy(x);
+  }
+  double ms(ptn_handle h) {
+    HIPCHK(hipEventSynchronize(e[1]));
+    float v = 0.f;
+    HIPCHK(hipEventElapsedTime(&v, e[0], e[1]));
+    return v;
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+ptn_status ptn_batch_set(ptn_handle h, int nprob, const int *dims, const int *ranks, const double *X) {
+  return run(h, [&] {
+    REQUIRE(X != nullptr, PTN_ERR_INVALID, "X is null");
+    batch_layout(h, nprob, dims, ranks);
+    BatchSet &b = *h->batch;
+    std::vector<float> xs(b.nx, 0.f);
+    int64_t src = 0;
+    for (const BatchDesc &d : b.desc) {
+      const int64_t n = batch_elems(d);
+      for (int64_t e = 0; e < n; ++e) xs[d.x_off + e] = (float)X[src + e];
+      src += n;
+    }
+    HIPCHK(hipMemcpyAsync(b.x32, xs.data(), b.nx * sizeof(float), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+  });
+}
+
+ptn_status ptn_batch_generate(ptn_handle h, int nprob, const int *dims, const int *ranks, uint64_t seed,
+                              double snr_db) {
+  return run(h, [&] {
+    batch_layout(h, nprob, dims, ranks);
+    BatchSet &b = *h->batch;
+    int64_t maxn = 1, maxf = 1;
+    for (const BatchDesc &d : b.desc) {
+      maxn = std::max(maxn, batch_elems(d));
+      maxf = std::max(maxf, (int64_t)(d.I[0] + d.I[1] + d.I[2]) * d.R);
+    }
+    DevBuf<double> x64(h, maxn), fac(h, maxf);
+    for (int p = 0; p < nprob; ++p) {
+      const BatchDesc &d = b.desc[p];
+      const int64_t dd[3] = {d.I[0], d.I[1], d.I[2]};
+      const int64_t n = batch_elems(d);
+      const uint64_t sp = seed + (uint64_t)p;
+      gen_factors(h->stream, 3, dd, d.R, sp, 0, fac.p);
+      cp_full(h->stream, 3, dd, d.R, fac.p, x64.p, nullptr);
+      if (!std::isnan(snr_db)) {
+        square_norm_partials(h->stream, n, x64.p, nullptr, h->part, kNormParts);
+        sum_partials(h->stream, h->part, kNormParts, h->scal + 1);
+        gen_noise_norm(h->stream, n, sp, h->part, kNormParts);
+        sum_partials(h->stream, h->part, kNormParts, h->scal + 2);
+        noise_scale(h->stream, h->scal + 1, h->scal + 2, snr_db, h->scal + 3);
+        add_noise(h->stream, n, sp, h->scal + 3, x64.p, nullptr);
+      }
+      cast_f64_to_f32(h->stream, n, x64.p, b.x32 + d.x_off);
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipGetLastError());
+  });
+}
+
+ptn_status ptn_batch_init_factors(ptn_handle h, uint64_t seed, double *packed_out) {
+  return run(h, [&] {
+    REQUIRE(h->batch, PTN_ERR_STATE, "no batch set on the handle");
+    REQUIRE(packed_out, PTN_ERR_INVALID, "output is null");
+    const BatchSet &b = *h->batch;
+    int64_t tot = 0;
+    for (const BatchDesc &d : b.desc) tot += (int64_t)(d.I[0] + d.I[1] + d.I[2]) * d.R;
+    DevBuf<double> fac(h, tot);
+    int64_t off = 0;
+    for (int p = 0; p < b.nprob; ++p) {
+      const BatchDesc &d = b.desc[p];
+      const int64_t dd[3] = {d.I[0], d.I[1], d.I[2]};
+      gen_factors(h->stream, 3, dd, d.R, seed + (uint64_t)p, 0, fac.p + off);
+      off += (int64_t)(d.I[0] + d.I[1] + d.I[2]) * d.R;
+    }
+    HIPCHK(hipMemcpyAsync(packed_out, fac.p, tot * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipGetLastError());
+  });
+}
+
+ptn_status ptn_batch_get_tensor(ptn_handle h, int p, double *X_out) {
+  return run(h, [&] {
+    REQUIRE(h->batch, PTN_ERR_STATE, "no batch set on the handle");
+    REQUIRE(p >= 0 && p < h->batch->nprob, PTN_ERR_INVALID, "no such problem");
+    REQUIRE(X_out, PTN_ERR_INVALID, "output is null");
+    const BatchDesc &d = h->batch->desc[p];
+    const int64_t n = batch_elems(d);
+    std::vector<float> xs(n);
+    HIPCHK(hipMemcpyAsync(xs.data(), h->batch->x32 + d.x_off, n * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    for (int64_t e = 0; e < n; ++e) X_out[e] = (double)xs[e];
+  });
+}
+
+ptn_status ptn_cpd_batch(ptn_handle h, const double *init_packed, const ptn_options *opt, double *out_packed,
+                         ptn_result *res, ptn_batch_log *log, int block, int grid, ptn_batch_report *rep) {
+  return run(h, [&] {
+    REQUIRE(h->batch, PTN_ERR_STATE, "no batch set on the handle");
+    REQUIRE(init_packed && opt && out_packed && res, PTN_ERR_INVALID, "null argument");
+    const ptn_options &o = *opt;
+    REQUIRE(o.max_iter >= 1 && o.max_iter <= 1000000, PTN_ERR_INVALID, "max_iter must be in [1, 1e6]");
+    REQUIRE(o.max_inner >= 0 && o.max_inner <= kMaxInner, PTN_ERR_UNSUPPORTED, "max_inner must be in [0, 255]");
+    REQUIRE(o.accel_coeff >= 1, PTN_ERR_INVALID, "accel_coeff must be >= 1");
+    REQUIRE(o.accel_fail >= 0, PTN_ERR_INVALID, "accel_fail must be >= 0");
+    for (int k = 0; k < 3; ++k)
+      REQUIRE(o.constraints[k] == PTN_NONNEGATIVITY, PTN_ERR_UNSUPPORTED, "only the nonnegativity constraint is built");
+    REQUIRE((block == 0 && grid == 0) || ((block == 256 || block == 512) && grid >= 1 && grid <= (1 << 16)),
+            PTN_ERR_INVALID, "block/grid: both 0 (autotune) or block 256 / 512 with grid in [1, 65536]");
+    const BatchSet &b = *h->batch;
+    const int P = b.nprob;
+    const int64_t nr = b.nrows * kBatchBR;
+    hipStream_t s = h->stream;
+
+    // factor rows (16 doubles, zero beyond R) from the packed column-major factors
+    std::vector<double> rows(nr, 0.0);
+    int64_t src = 0;
+    for (const BatchDesc &d : b.desc) {
+      for (int k = 0; k < 3; ++k) {
+        const int I = d.I[k];
+        for (int r = 0; r < d.R; ++r)
+          for (int i = 0; i < I; ++i) rows[(d.f_off + d.rowoff[k] + i) * kBatchBR + r] = init_packed[src + i + (int64_t)I * r];
+        src += (int64_t)I * d.R;
+      }
+    }
+    std::vector<BatchProb> st0(P);
+    for (BatchProb &q : st0) {
+      std::memset(&q, 0, sizeof(q));
+      q.accel_coeff = o.accel_coeff;
+      q.accel_fail = o.accel_fail;
+    }
+    // largest work first (tensor elements times the padded rank), ties by index
+    std::vector<int> order(P);
+    std::iota(order.begin(), order.end(), 0);
+    auto work = [&](int p) {
+      const BatchDesc &d = b.desc[p];
+      return batch_elems(d) * ((d.R + 3) / 4 * 4 + 4);
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work(x) > work(y); });
+    // (sweep + 1)^(1 / coeff) for every sweep and every coefficient the bookkeeping can reach
+    const int stride = o.accel_coeff + o.max_iter / 5 + 3;
+    std::vector<double> pw((int64_t)(o.max_iter + 1) * stride, 0.0);
+    for (int sw = 0; sw <= o.max_iter; ++sw)
+      for (int c = 1; c < stride; ++c) pw[(int64_t)sw * stride + c] = std::pow((double)(sw + 1), 1.0 / (double)c);
+
+    DevBuf<BatchDesc> desc(h, P);
+    DevBuf<BatchProb> st(h, P), st_snap(h, P);
+    DevBuf<double> fac(h, nr), old(h, nr), acc(h, nr), grams(h, (size_t)P * 3 * kBatchBR * kBatchBR);
+    DevBuf<double> fac_snap(h, nr), old_snap(h, nr), grams_snap(h, grams.n), powt(h, pw.size());
+    DevBuf<int> ord(h, P), queue(h, o.max_iter + 2), ctr(h, 1);
+    DevBuf<BatchLog> lg(h, log ? (size_t)P * o.max_iter : 1);
+    const int one = 1;
+    HIPCHK(hipMemcpyAsync(desc.p, b.desc.data(), P * sizeof(BatchDesc), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(st.p, st0.data(), P * sizeof(BatchProb), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(fac.p, rows.data(), nr * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(ord.p, order.data(), P * sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(powt.p, pw.data(), pw.size() * sizeof(double), hipMemcpyHostToDevice, s));
+
+    BatchArgs a{};
+    a.nprob = P;
+    a.desc = desc.p;
+    a.st = st.p;
+    a.X = b.x32;
+    a.fac = fac.p;
+    a.old = old.p;
+    a.acc = acc.p;
+    a.grams = grams.p;
+    a.order = ord.p;
+    a.queue = queue.p;
+    a.sweep_ctr = ctr.p;
+    a.max_iter = o.max_iter;
+    a.thr = o.threshold_error;
+    a.d1 = o.nesterov_delta_1;
+    a.d2 = o.nesterov_delta_2;
+    a.max_inner = o.max_inner;
+    a.acceleration = o.acceleration;
+    a.normalization = o.normalization;
+    a.pow_table = powt.p;
+    a.pow_stride = stride;
+    a.log = nullptr;
+    batch_init(s, a);                                   // ||X_p||^2 and the initial Grams
+
+    int cu = 0;
+    HIPCHK(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, h->device));
+    auto restart = [&] {
+      HIPCHK(hipMemsetAsync(queue.p, 0, queue.n * sizeof(int), s));
+      HIPCHK(hipMemcpyAsync(ctr.p, &one, sizeof(int), hipMemcpyHostToDevice, s));
+    };
+    double tune_s = 0.0;
+    const bool tune = block == 0;
+    if (tune) {
+      // autotuner: a few launch shapes, each timed over the first two sweeps from the
+      // same state (saved here and restored after every candidate)
+      const auto t0 = std::chrono::steady_clock::now();
+      HIPCHK(hipMemcpyAsync(st_snap.p, st.p, P * sizeof(BatchProb), hipMemcpyDeviceToDevice, s));
+      HIPCHK(hipMemcpyAsync(fac_snap.p, fac.p, nr * sizeof(double), hipMemcpyDeviceToDevice, s));
+      HIPCHK(hipMemcpyAsync(old_snap.p, old.p, nr * sizeof(double), hipMemcpyDeviceToDevice, s));
+      HIPCHK(hipMemcpyAsync(grams_snap.p, grams.p, grams.n * sizeof(double), hipMemcpyDeviceToDevice, s));
+      auto restore = [&] {
+        HIPCHK(hipMemcpyAsync(st.p, st_snap.p, P * sizeof(BatchProb), hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(fac.p, fac_snap.p, nr * sizeof(double), hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(old.p, old_snap.p, nr * sizeof(double), hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(grams.p, grams_snap.p, grams.n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        restart();
+      };
+      const int cand[][2] = {{256, 2 * cu}, {256, cu}, {512, cu}, {512, cu / 2 > 0 ? cu / 2 : 1}};
+      const int tsweeps = std::min(2, o.max_iter);
+      double best = 1e300;
+      TimingEvents ev(h);
+      for (const auto &c : cand) {
+        restore();
+        HIPCHK(hipEventRecord(ev.e[0], s));
+        for (int k = 0; k < tsweeps; ++k) {
+          batch_sweep(s, a, c[0], c[1]);
+          batch_next_sweep(s, ctr.p);
+        }
+        HIPCHK(hipEventRecord(ev.e[1], s));
+        const double ms = ev.ms(h);
+        HIPCHK(hipGetLastError());
+        if (ms < best) {
+          best = ms;
+          block = c[0];
+          grid = c[1];
+        }
+      }
+      restore();
+      HIPCHK(hipStreamSynchronize(s));
+      tune_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    } else {
+      restart();
+    }
+
+    // one sweep captured once (the kernel reads the sweep index from the device counter
+    // that the second node bumps), replayed max_iter times
+    if (log) {
+      HIPCHK(hipMemsetAsync(lg.p, 0, lg.n * sizeof(BatchLog), s));
+      a.log = lg.p;
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    GraphExec gx;
+    HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    batch_sweep(s, a, block, grid);
+    batch_next_sweep(s, ctr.p);
+    HIPCHK(hipStreamEndCapture(s, &gx.g));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipGraphInstantiate(&gx.e, gx.g, nullptr, nullptr, 0));
+    TimingEvents ev(h);
+    HIPCHK(hipEventRecord(ev.e[0], s));
+    for (int it = 0; it < o.max_iter; ++it) HIPCHK(hipGraphLaunch(gx.e, s));
+    HIPCHK(hipEventRecord(ev.e[1], s));
+    const double ms = ev.ms(h);
+    HIPCHK(hipGetLastError());
+
+    std::vector<BatchProb> stf(P);
+    std::vector<BatchDesc> df(P);
+    HIPCHK(hipMemcpyAsync(stf.data(), st.p, P * sizeof(BatchProb), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(df.data(), desc.p, P * sizeof(BatchDesc), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(rows.data(), fac.p, nr * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (log) HIPCHK(hipMemcpyAsync(log, lg.p, lg.n * sizeof(BatchLog), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    int64_t dst = 0;
+    for (int p = 0; p < P; ++p) {
+      const BatchDesc &d = b.desc[p];
+      for (int k = 0; k < 3; ++k) {
+        const int I = d.I[k];
+        for (int r = 0; r < d.R; ++r)
+          for (int i = 0; i < I; ++i) out_packed[dst + i + (int64_t)I * r] = rows[(d.f_off + d.rowoff[k] + i) * kBatchBR + r];
+        dst += (int64_t)I * d.R;
+      }
+      ptn_result &rr = res[p];
+      std::memset(&rr, 0, sizeof(rr));
+      rr.frob_sq = df[p].frob;
+      rr.f_value = stf[p].f;
+      rr.rel_cost = stf[p].f / std::sqrt(df[p].frob);
+      rr.ao_iter = stf[p].sweeps;
+      rr.accel_coeff = stf[p].accel_coeff;
+      rr.accel_fail = stf[p].accel_fail;
+      rr.sweep_seconds = ms * 1e-3;
+    }
+    if (rep) {
+      rep->block = block;
+      rep->grid = grid;
+      rep->tuned = tune ? 1 : 0;
+      rep->seconds = ms * 1e-3;
+      rep->tune_seconds = tune_s;
+      rep->sweeps = o.max_iter;
     }
   });
 }

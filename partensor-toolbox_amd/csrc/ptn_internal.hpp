@@ -34,6 +34,55 @@ struct NnlsParams {
   int fallback;         // zero-column blend applied
 };
 
+// ---- batched small CPDs (batch_kernels.hip) ----
+// One order-3 problem of a batch: sizes, rank (<= 16), where its tensor, factor rows
+// (rows of 16 doubles, factors 0, 1, 2 back to back) and Grams (3 x 16 x 16) live.
+struct BatchDesc {
+  int I[3];
+  int R;
+  int rowoff[3];
+  int pad;
+  int64_t x_off, f_off, g_off;
+  double frob;                 // ||X||^2, set by batch_init
+};
+// ALS state of one problem, kept on the device between sweeps.
+struct BatchProb {
+  double f, f_accel;
+  int accel_coeff, accel_fail;
+  int done;                    // stopped on the threshold
+  int sweeps;                  // sweeps completed
+  int tried, accepted;         // line search of the last sweep
+  int nnls_iters[3];
+  int pad;
+};
+// One record per problem and sweep (layout of ptn_batch_log in the C ABI).
+struct BatchLog {
+  double f_value, f_accel;
+  int nnls_iters[3];
+  int accel_tried, accel_accepted, stopped;
+};
+struct BatchArgs {
+  static constexpr int kMaxThreads = 512;
+  int nprob;
+  BatchDesc *desc;
+  BatchProb *st;
+  const float *X;
+  double *fac, *old, *acc, *grams;   // factor rows (current, previous sweep, extrapolated), Grams
+  const int *order;            // problems, largest first
+  int *queue;                  // one work counter per sweep (zero-initialised)
+  const int *sweep_ctr;        // the sweep being run (1-based), bumped after each sweep
+  int max_iter;
+  double thr, d1, d2;
+  int max_inner, acceleration, normalization;
+  const double *pow_table;     // (sweep + 1)^(1 / coeff) at [sweep * pow_stride + coeff]
+  int pow_stride;
+  BatchLog *log;               // [nprob][max_iter] (optional)
+};
+size_t batch_lds_bytes();
+void batch_init(hipStream_t s, const BatchArgs &a);
+void batch_sweep(hipStream_t s, const BatchArgs &a, int block, int grid);
+void batch_next_sweep(hipStream_t s, int *ctr);
+
 // ---- generation (gen_kernels.hip) ----
 void gen_factors(hipStream_t s, int N, const int64_t *dims, int R, uint64_t seed, int dist, double *packed);
 void cp_full(hipStream_t s, int N, const int64_t *dims, int R, const double *packed_dev,

@@ -15,27 +15,11 @@ namespace {
 struct GramPtrs {
   const double *p[kMaxOrder];
 };
+#include "wave_common.inc"
+
 struct MutPtrs {
   double *p[kMaxOrder];
 };
-
-constexpr double kEps = 2.220446049250313e-16;
-
-__device__ __forceinline__ double bcast_lane(double v, int k) {
-  int2 p = __builtin_bit_cast(int2, v);
-  p.x = __builtin_amdgcn_readlane(p.x, k);
-  p.y = __builtin_amdgcn_readlane(p.y, k);
-  return __builtin_bit_cast(double, p);
-}
-
-__device__ __forceinline__ double wave_max(double v) {
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-  return v;
-}
-__device__ __forceinline__ double wave_min(double v) {
-  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
-  return v;
-}
 
 // Accumulate in list order: V = G_l0; V = G_lj * V (right-nested when the list is
 // ascending, left-nested along a tree path when given in path order).
@@ -102,102 +86,15 @@ __global__ __launch_bounds__(256) void k_gram_combine(int R, int nchunks, const 
   }
 }
 
-constexpr double kPivminBase = 2.2250738585072014e-308;
-
-__device__ int sturm_count(int n, const double *d, const double *e2, double pivmin, double x) {
-  int c = 0;
-  double q = d[0] - x;
-  if (fabs(q) < pivmin) q = -pivmin;
-  if (q < 0.0) ++c;
-  for (int i = 1; i < n; ++i) {
-    q = (d[i] - x) - e2[i - 1] / q;
-    if (fabs(q) < pivmin) q = -pivmin;
-    if (q < 0.0) ++c;
-  }
-  return c;
-}
-
-__device__ __forceinline__ double update_alpha(double alpha, double q) {
-  double a = 1.0;
-  double b = alpha * alpha - q;
-  double c = -alpha * alpha;
-  double D = b * b - 4.0 * a * c;
-  return (-b + sqrt(D)) / 2.0;
-}
-
-// Multisection for the j-th smallest eigenvalue, run by one 32-lane half of the wave:
-// lane m < 31 probes lo + (m+1)(hi-lo)/32 each step; the new bracket is the first
-// probe interval whose count exceeds j (the oracle scans the probes in order).  Lane
-// 31, idle otherwise, counts the eigenvalues below 0 in the first step (*k0).
-__device__ double sturm_find_half(int n, const double *d, const double *e2, double pivmin, double lo, double hi,
-                                  int j, int m, int half, int *k0) {
-  for (int it = 0; it < 64; ++it) {
-    const double w = hi - lo;
-    const double sc = fabs(lo) > fabs(hi) ? fabs(lo) : fabs(hi);
-    const bool stop = w <= 2.0 * kEps * sc || w <= pivmin;
-    const bool probe = m < 31 ? !stop : (k0 != nullptr && it == 0);
-    const double x = m < 31 ? fma(w, (double)(m + 1) * 0.03125, lo) : 0.0;
-    const int cnt = probe ? sturm_count(n, d, e2, pivmin, x) : 0;
-    if (k0 && it == 0 && m == 31) *k0 = cnt;
-    const unsigned long long bal = __ballot(m < 31 && probe && cnt > j);
-    const unsigned bits = (unsigned)(bal >> (32 * half)) & 0x7fffffffu;
-    if (stop) break;
-    const int ms = bits ? __builtin_ctz(bits) : 31;
-    const double nlo = ms > 0 ? fma(w, (double)ms * 0.03125, lo) : lo;
-    const double nhi = ms < 31 ? fma(w, (double)(ms + 1) * 0.03125, lo) : hi;
-    if (nlo == lo && nhi == hi) break;
-    lo = nlo;
-    hi = nhi;
-  }
-  return 0.5 * (lo + hi);
-}
-
-// Cross-lane move of a double through DPP (both halves; lanes without a source read 0,
-// lanes outside ROWMASK read 0).
-template <int CTRL, int ROWMASK = 0xf>
-__device__ __forceinline__ double dpp_d(double x) {
-  int2 v = __builtin_bit_cast(int2, x);
-  v.x = __builtin_amdgcn_update_dpp(0, v.x, CTRL, ROWMASK, 0xf, true);
-  v.y = __builtin_amdgcn_update_dpp(0, v.y, CTRL, ROWMASK, 0xf, true);
-  return __builtin_bit_cast(double, v);
-}
-
-// Sum over the 64 lanes as a balanced tree over consecutive lanes,
-// ((x0 + x1) + (x2 + x3)) + ...: quad swaps, row shifts by 4 and 8, then the row
-// broadcasts; every addition has the operands of the tree's node (in either order,
-// which IEEE addition does not distinguish).  The total lands in lane 63 and is
-// returned to every lane.
-__device__ __forceinline__ double tree64(double x) {
-  x = x + dpp_d<0xB1>(x);          // quad_perm [1,0,3,2]
-  x = x + dpp_d<0x4E>(x);          // quad_perm [2,3,0,1]
-  x = x + dpp_d<0x114>(x);         // row_shr:4  (lane 4q+3 of each 8 sums its 8)
-  x = x + dpp_d<0x118>(x);         // row_shr:8  (lane 15 of each row sums the row)
-  x = x + dpp_d<0x142, 0xA>(x);    // row_bcast:15 into rows 1, 3
-  x = x + dpp_d<0x143, 0xC>(x);    // row_bcast:31 into rows 2, 3
-  return bcast_lane(x, 63);
-}
-
-// Extreme singular values (= max / min |eigenvalue|) of the symmetric V: Householder
-// tridiagonalization (Golub & Van Loan Alg. 8.3.1) by one wave with the matrix in LDS,
-// then Sturm multisection (Alg. 8.4.1), the two 32-lane halves bracketing two
-// eigenvalues at once.  Lane l owns index l.  The working matrix t holds V as loaded
-// (t[r][c] = V[r][c]); step k reads row k as the column to eliminate and forms
-// p_l = sum_j t[j][l] v_j, so every LDS access is lane-contiguous or a broadcast.
-// Sums: 64-lane trees (tree64) for sigma and p.v, four interleaved chains over the RB
-// padded rows for the matrix-vector product; the CPU restatement performs the same
-// operations in the same order, padding included.
 // With ng > 0 the matrix is first formed as the Hadamard product of the ng Grams in
 // list order (V = G_l0, V = G_lj * V; PartialCwiseProd) and stored to Vout; with
 // betas != nullptr the momentum coefficients of the Nesterov solve follow the
-// eigenvalues (one kernel prepares an NNLS update).
+// eigenvalues (one kernel prepares an NNLS update).  One wave.
 template <int RB>
 __global__ __launch_bounds__(64) void k_eig_extremes(int R, const double *__restrict__ V, GramPtrs G, int ng,
                                                      double *__restrict__ Vout, NnlsParams *prm, int max_inner,
                                                      double *__restrict__ betas) {
-  __shared__ double t[RB * RB];
-  __shared__ double vb[RB], pb[RB], d[RB], e[RB], e2[RB];
-  __shared__ double sh_res[4];
-  __shared__ int sh_k0;
+  __shared__ EigLds<RB> S;
   const int n = R, l = threadIdx.x;
   for (int x = l; x < RB * RB; x += 64) {
     const int r = x / RB, c = x % RB;
@@ -212,113 +109,14 @@ __global__ __launch_bounds__(64) void k_eig_extremes(int R, const double *__rest
         v = V[e];
       }
     }
-    t[x] = v;
-  }
-  if (l < RB) { e[l] = 0.0; vb[l] = 0.0; pb[l] = 0.0; }
-  __syncthreads();
-  const bool own = l < RB;
-  for (int k = 0; k + 2 < n; ++k) {
-    const double xi = (l >= k + 2 && l < n) ? t[k * RB + l] : 0.0;
-    const double x0 = t[k * RB + k + 1];
-    const double sigma = tree64(xi * xi);
-    if (sigma == 0.0) {                                   // wave-uniform
-      if (l == 0) e[k] = x0;
-      continue;
-    }
-    const double mu = sqrt(fma(x0, x0, sigma));
-    const double v0 = x0 <= 0.0 ? x0 - mu : x0 + mu;
-    if (l == 0) e[k] = x0 <= 0.0 ? mu : -mu;
-    const double beta = 2.0 / fma(v0, v0, sigma);
-    const double vi = l == k + 1 ? v0 : xi;
-    if (own) vb[l] = vi;
-    __syncthreads();
-    double s4[4] = {0.0, 0.0, 0.0, 0.0};
-    if (own) {
-#pragma unroll
-      for (int j = 0; j < RB; ++j) s4[j & 3] = fma(t[j * RB + l], vb[j], s4[j & 3]);
-    }
-    const bool act = l > k && l < n;
-    double pi = act ? beta * ((s4[0] + s4[1]) + (s4[2] + s4[3])) : 0.0;
-    const double K = (0.5 * beta) * tree64(pi * vi);
-    if (act) pi = fma(-K, vi, pi);
-    if (own) pb[l] = pi;
-    __syncthreads();
-    if (act) {
-#pragma unroll 8
-      for (int j = k + 1; j < n; ++j) t[j * RB + l] = t[j * RB + l] - (vi * pb[j] + pi * vb[j]);
-    }
-    __syncthreads();
-  }
-  if (l < n) d[l] = t[l * RB + l];
-  if (l == 0 && n >= 2) e[n - 2] = t[(n - 2) * RB + (n - 1)];
-  __syncthreads();
-  // Gershgorin bracket and pivmin (lane-parallel; max / min are order-independent)
-  const double el = l + 1 < n ? e[l] : 0.0;
-  const double e2l = el * el;
-  if (l + 1 < n) e2[l] = e2l;
-  const double pivmin = kPivminBase * fmax(1.0, wave_max(e2l));
-  double glo = INFINITY, ghi = -INFINITY;
-  if (l < n) {
-    const double r = (l > 0 ? fabs(e[l - 1]) : 0.0) + fabs(el);
-    glo = d[l] - r;
-    ghi = d[l] + r;
-  }
-  glo = wave_min(glo);
-  ghi = wave_max(ghi);
-  const double pad = 2.0 * kEps * (fabs(glo) > fabs(ghi) ? fabs(glo) : fabs(ghi)) + pivmin;
-  glo -= pad;
-  ghi += pad;
-  __syncthreads();
-  {
-    const int half = l >> 5, mm = l & 31;
-    int k0l = 0;
-    const double r1 = sturm_find_half(n, d, e2, pivmin, glo, ghi, half == 0 ? 0 : n - 1, mm, half, &k0l);
-    const int k0 = __builtin_amdgcn_readlane(k0l, 31);
-    double r2 = 0.0;
-    const bool inner = k0 > 0 && k0 < n;                   // wave-uniform
-    if (inner) r2 = sturm_find_half(n, d, e2, pivmin, glo, ghi, half == 0 ? k0 - 1 : k0, mm, half, nullptr);
-    if (mm == 0) {
-      sh_res[half] = r1;
-      sh_res[2 + half] = r2;
-    }
-    if (l == 0) sh_k0 = k0;
+    S.t[x] = v;
   }
   __syncthreads();
-  if (l == 0) {
-    const double lmin = sh_res[0], lmax = sh_res[1];
-    const int k0 = sh_k0;
-    double amin;
-    if (k0 == 0) amin = fabs(lmin);
-    else if (k0 == n) amin = fabs(lmax);
-    else {
-      const double a1 = fabs(sh_res[2]), a2 = fabs(sh_res[3]);
-      amin = a1 < a2 ? a1 : a2;
-    }
-    const double amax = fabs(lmin) > fabs(lmax) ? fabs(lmin) : fabs(lmax);
-    // GLambda, NesterovMNLS.hpp:75-91
-    double L = amax, mu = amin, lambda;
-    double q = mu / L;
-    if (1.0 / q > 1e6) lambda = 10.0 * mu;
-    else if (1.0 / q > 1e3) lambda = mu;
-    else lambda = mu / 10.0;
-    L += lambda;
-    mu += lambda;
-    q = mu / L;
-    prm->L0 = amax;
-    prm->mu0 = amin;
-    prm->L = L;
-    prm->lambda = lambda;
-    prm->q = q;
-    if (betas) {                                   // NesterovMNLS.hpp:222-223
-      double alpha = 1.0;
-      for (int k = 0; k < max_inner; ++k) {
-        const double na = update_alpha(alpha, q);
-        betas[k] = alpha * (1.0 - alpha) / (alpha * alpha + na);
-        alpha = na;
-      }
-    }
-  }
+  double amax, amin;
+  eig_extremes_wave<RB>(n, S, amax, amin);
+  if (l == 0) glambda_betas(amax, amin, prm, max_inner, betas);
 }
+
 
 
 // One 64-lane wave per factor row; lane r holds column r and keeps column r of

@@ -29,10 +29,12 @@ EXPORTED = [
     "ptn_set_tensor", "ptn_generate_tensor", "ptn_generate_factors", "ptn_tensor_norm_sq",
     "ptn_cpd", "ptn_mttkrp", "ptn_mttkrp_dimtree", "ptn_nesterov_mnls", "ptn_sym_extremes",
     "ptn_normalize", "ptn_cost_explicit", "ptn_time_contraction",
+    "ptn_batch_set", "ptn_batch_generate", "ptn_batch_init_factors", "ptn_batch_get_tensor", "ptn_cpd_batch",
 ]
 
 _d, _i, _i64, _u64 = ctypes.c_double, ctypes.c_int, ctypes.c_int64, ctypes.c_uint64
 _pd = ctypes.POINTER(ctypes.c_double)
+_pi = ctypes.POINTER(ctypes.c_int)
 _pi64 = ctypes.POINTER(ctypes.c_int64)
 _vp = ctypes.c_void_p
 
@@ -57,6 +59,16 @@ class SweepLog(ctypes.Structure):
         ("nnls_iters", _i * MAX_ORDER), ("zero_col_fallback", _i * MAX_ORDER),
         ("lipschitz", _d * MAX_ORDER), ("mu", _d * MAX_ORDER), ("t_end", _d),
     ]
+
+
+class BatchLog(ctypes.Structure):
+    _fields_ = [("f_value", _d), ("f_accel", _d), ("nnls_iters", _i * 3), ("accel_tried", _i),
+                ("accel_accepted", _i), ("stopped", _i)]
+
+
+class BatchReport(ctypes.Structure):
+    _fields_ = [("block", _i), ("grid", _i), ("tuned", _i), ("sweeps", _i), ("seconds", _d),
+                ("tune_seconds", _d)]
 
 
 class PtnError(RuntimeError):
@@ -95,6 +107,12 @@ def lib():
         L.ptn_normalize.argtypes = [_vp, _i, _pi64, _i, _i, _pd, _pd]
         L.ptn_cost_explicit.argtypes = [_vp, _i, _pd, _pd]
         L.ptn_time_contraction.argtypes = [_vp, _i, _i, _pd, _i, _i, _pd, _pd]
+        L.ptn_batch_set.argtypes = [_vp, _i, _pi, _pi, _pd]
+        L.ptn_batch_generate.argtypes = [_vp, _i, _pi, _pi, _u64, _d]
+        L.ptn_batch_init_factors.argtypes = [_vp, _u64, _pd]
+        L.ptn_batch_get_tensor.argtypes = [_vp, _i, _pd]
+        L.ptn_cpd_batch.argtypes = [_vp, _pd, ctypes.POINTER(Options), _pd, ctypes.POINTER(Result),
+                                    ctypes.POINTER(BatchLog), _i, _i, ctypes.POINTER(BatchReport)]
         _lib = L
     return _lib
 
@@ -212,6 +230,81 @@ class Handle:
     def cpd_dimtree(self, R, init, **kw):
         kw.setdefault("mttkrp_form", MTTKRP_DIMTREE)
         return self.cpd(R, init, **kw)
+
+    # -- batched independent order-3 problems --
+    def _set_batch_shape(self, dims, ranks):
+        d = np.ascontiguousarray(np.asarray(dims, dtype=np.int32).reshape(-1, 3))
+        r = np.ascontiguousarray(np.asarray(ranks, dtype=np.int32).ravel())
+        if d.shape[0] != r.shape[0]:
+            raise ValueError("dims and ranks describe different numbers of problems")
+        self.batch_dims, self.batch_ranks = [tuple(int(v) for v in x) for x in d], [int(v) for v in r]
+        return d, r
+
+    def batch_set(self, tensors, ranks):
+        """Upload a batch: a list of order-3 arrays (stored FP32 on the device)."""
+        dims = [t.shape for t in tensors]
+        d, r = self._set_batch_shape(dims, ranks)
+        flat = np.ascontiguousarray(np.concatenate([np.asarray(t, dtype=np.float64).ravel(order="F")
+                                                    for t in tensors]))
+        self._check(lib().ptn_batch_set(self._h, d.shape[0], d.ctypes.data_as(_pi), r.ctypes.data_as(_pi),
+                                        _p(flat)))
+
+    def batch_generate(self, dims, ranks, seed, snr_db=None):
+        d, r = self._set_batch_shape(dims, ranks)
+        snr = float("nan") if snr_db is None else float(snr_db)
+        self._check(lib().ptn_batch_generate(self._h, d.shape[0], d.ctypes.data_as(_pi), r.ctypes.data_as(_pi),
+                                             seed, snr))
+
+    def batch_init_factors(self, seed):
+        """Initial factors of every problem: a list (per problem) of [A0, A1, A2]."""
+        n = sum(sum(dd) * R for dd, R in zip(self.batch_dims, self.batch_ranks))
+        out = np.empty(n, dtype=np.float64)
+        self._check(lib().ptn_batch_init_factors(self._h, seed, _p(out)))
+        return self._unpack_batch(out)
+
+    def batch_tensor(self, p):
+        dd = self.batch_dims[p]
+        out = np.empty(dd[0] * dd[1] * dd[2], dtype=np.float64)
+        self._check(lib().ptn_batch_get_tensor(self._h, p, _p(out)))
+        return out.reshape(dd, order="F")
+
+    def _unpack_batch(self, flat):
+        res, o = [], 0
+        for dd, R in zip(self.batch_dims, self.batch_ranks):
+            n = sum(dd) * R
+            res.append(unpack(flat[o:o + n], dd, R))
+            o += n
+        return res
+
+    def cpd_batch(self, init, options=None, block=0, grid=0, with_log=True, **kw):
+        """ALS on every problem of the batch.  Returns (factors per problem, status dict per
+        problem, per-problem sweep logs or None, report dict)."""
+        o = options if options is not None else default_options(**kw)
+        P = len(self.batch_ranks)
+        flat = np.ascontiguousarray(np.concatenate([pack(f) for f in init]))
+        out = np.empty_like(flat)
+        res = (Result * P)()
+        logs = (BatchLog * (P * int(o.max_iter)))() if with_log else None
+        rep = BatchReport()
+        self._check(lib().ptn_cpd_batch(self._h, _p(flat), ctypes.byref(o), _p(out), res, logs, block, grid,
+                                        ctypes.byref(rep)))
+        status = [{"frob_sq": r.frob_sq, "f_value": r.f_value, "rel_cost": r.rel_cost, "ao_iter": r.ao_iter,
+                   "accel_coeff": r.accel_coeff, "accel_fail": r.accel_fail} for r in res]
+        log = None
+        if with_log:
+            m = int(o.max_iter)
+            log = []
+            for p in range(P):
+                lp = []
+                for i in range(min(res[p].ao_iter, m)):
+                    r = logs[p * m + i]
+                    lp.append({"f_value": r.f_value, "f_accel": r.f_accel, "nnls_iters": list(r.nnls_iters),
+                               "accel_tried": r.accel_tried, "accel_accepted": r.accel_accepted,
+                               "stopped": r.stopped})
+                log.append(lp)
+        report = {"block": rep.block, "grid": rep.grid, "tuned": rep.tuned, "sweeps": rep.sweeps,
+                  "seconds": rep.seconds, "tune_seconds": rep.tune_seconds}
+        return self._unpack_batch(out), status, log, report
 
     # -- single operations --
     def mttkrp(self, factors, mode, arith=ARITH_F64):

@@ -1,0 +1,184 @@
+"""Batched independent CPDs on the GPU against per-problem runs of the CPU oracle.
+
+The batch has no reference counterpart (the reference runs one cpd() per tensor); it is
+pinned by running the oracle's cpd() (standard MTTKRP, Gram-form cost) on every problem
+with the same FP32-rounded tensor and the same initial factors.  The MTTKRP summation
+order differs from the oracle's, so costs and factors agree to the tolerances written
+below, while every discrete decision (Nesterov step counts, accepted line searches,
+threshold stops) must agree exactly."""
+import numpy as np
+import pytest
+
+import oracle as o
+import ptn_hip as P
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-test]", *a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def h():
+    say("create handle")
+    with P.Handle(0) as hd:
+        yield hd
+
+
+# mode sizes on both sides of the lane-mapping branches (I0 < 64, = 64, > 64, = 128),
+# ranks 1..16 across the four rank paddings, a few tiny modes
+EDGE = [((128, 128, 128), 16), ((64, 65, 63), 13), ((16, 17, 18), 4), ((100, 37, 90), 9), ((2, 3, 4), 2),
+        ((33, 128, 16), 16), ((127, 5, 64), 7), ((20, 20, 20), 1), ((48, 80, 112), 12), ((65, 64, 63), 8),
+        ((17, 120, 33), 5), ((90, 16, 128), 11)]
+
+
+def _problems(n_random, seed):
+    rng = np.random.default_rng(seed)
+    probs = list(EDGE)
+    for _ in range(n_random):
+        probs.append((tuple(int(v) for v in rng.integers(2, 129, 3)), int(rng.integers(1, 17))))
+    return probs
+
+
+def _tensors(probs, snr_db=25.0, seed0=0):
+    """FP64 tensors from the oracle's generator, rounded to FP32 (what the device stores)."""
+    Xs = []
+    for p, (dims, R) in enumerate(probs):
+        X, _ = o.make_tensor(list(dims), R, seed=seed0 + p, snr_db=snr_db)
+        Xs.append(np.asfortranarray(X.astype(np.float32).astype(np.float64)))
+    return Xs
+
+
+def _check_problem(p, Fg, stg, logg, Fo, sto, logo, ftol, atol):
+    """Sweep by sweep: equal step counts and line-search outcomes, costs within ftol.  A
+    line search whose two costs tie to 1e-9 in the oracle may go either way (the costs
+    carry ~1e-13 of summation-order difference): the comparison ends at such a sweep,
+    and the problem must still run the same number of sweeps.  Without a tie the final
+    factors agree within atol."""
+    assert stg["ao_iter"] == sto["ao_iter"], p
+    tie = None
+    for k, (a, b) in enumerate(zip(logg, logo)):
+        assert a["f_value"] == pytest.approx(b["f_value"], rel=ftol), (p, k)
+        assert a["nnls_iters"] == b["nnls_iters"][:3], (p, k)
+        assert a["accel_tried"] == b["accel_tried"], (p, k)
+        if b["accel_tried"]:
+            assert a["f_accel"] == pytest.approx(b["f_accel"], rel=ftol), (p, k)
+        if a["accel_accepted"] != b["accel_accepted"]:
+            assert abs(b["f_value"] - b["f_accel"]) <= 1e-9 * b["f_value"], (p, k, b["f_value"], b["f_accel"])
+            tie = k
+            break
+    if tie is not None:
+        say("problem", p, "line-search tie at sweep", tie + 1, "- compared up to there")
+        return
+    assert (stg["accel_coeff"], stg["accel_fail"]) == (sto["accel_coeff"], sto["accel_fail"]), p
+    assert stg["f_value"] == pytest.approx(sto["f_value"], rel=ftol), p
+    for a, b in zip(Fg, Fo):
+        assert P.rel_err(a, b) < atol, (p, P.rel_err(a, b))
+
+
+def test_batch_parity_vs_oracle(h):
+    probs = _problems(12, seed=11)
+    say("batch parity:", len(probs), "problems")
+    Xs = _tensors(probs)
+    init = [o.gen_factors(list(d), R, seed=100 + p) for p, (d, R) in enumerate(probs)]
+    h.batch_set(Xs, [R for _, R in probs])
+    kw = dict(max_iter=12, threshold_error=0.0)
+    Fg, stg, logg, rep = h.cpd_batch(init, block=256, grid=40, **kw)
+    say("report", rep)
+    for p, ((dims, R), X) in enumerate(zip(probs, Xs)):
+        Fo, sto, logo = o.cpd(X, R, init[p], cost_form=o.COST_GRAM, **kw)
+        _check_problem(p, Fg[p], stg[p], logg[p], Fo, sto, logo, 1e-10, 1e-9)
+        assert stg[p]["frob_sq"] == pytest.approx(sto["frob_sq"], rel=1e-13)
+
+
+def test_batch_launch_shapes_agree_bitwise(h):
+    """Each problem is solved by one block in a fixed order of operations, so the grid
+    never changes a result: one block size on different grids (and the autotuned shape,
+    for its block size) gives identical bits."""
+    probs = _problems(20, seed=12)
+    say("launch shapes:", len(probs), "problems")
+    Xs = _tensors(probs, seed0=50)
+    init = [o.gen_factors(list(d), R, seed=200 + p) for p, (d, R) in enumerate(probs)]
+    h.batch_set(Xs, [R for _, R in probs])
+    kw = dict(max_iter=8, threshold_error=0.0)
+    runs = {}
+    for shape in [(256, 1), (256, 7), (256, 512), (512, 3), (512, 256), (0, 0)]:
+        F, st, _, rep = h.cpd_batch(init, block=shape[0], grid=shape[1], with_log=False, **kw)
+        say("shape", shape, "->", (rep["block"], rep["grid"]), "seconds", rep["seconds"])
+        runs[shape] = (rep["block"], F, [s["f_value"] for s in st])
+    for bs in (256, 512):
+        same = [v for v in runs.values() if v[0] == bs]
+        for v in same[1:]:
+            assert v[2] == same[0][2]
+            for Fa, Fb in zip(v[1], same[0][1]):
+                for a, b in zip(Fa, Fb):
+                    assert np.array_equal(a, b)
+    # the two block sizes sum the mode-0 partials and the cost terms over different
+    # thread counts: equal to rounding
+    a256 = [v for v in runs.values() if v[0] == 256][0]
+    a512 = [v for v in runs.values() if v[0] == 512][0]
+    assert a256[2] == pytest.approx(a512[2], rel=1e-11)
+
+
+def test_batch_threshold_stop(h):
+    """Problems stop on their own threshold test at the oracle's sweep and are left alone
+    afterwards; the stopping sweep takes no line search."""
+    probs = [((40, 30, 20), 4), ((60, 50, 40), 6), ((25, 25, 25), 3), ((70, 20, 30), 5)]
+    Xs = _tensors(probs, snr_db=30.0, seed0=8)
+    init = [o.gen_factors(list(d), R, seed=9 + p) for p, (d, R) in enumerate(probs)]
+    h.batch_set(Xs, [R for _, R in probs])
+    kw = dict(max_iter=120, threshold_error=0.05)
+    Fg, stg, logg, _ = h.cpd_batch(init, block=256, grid=2, **kw)
+    for p, ((dims, R), X) in enumerate(zip(probs, Xs)):
+        Fo, sto, logo = o.cpd(X, R, init[p], cost_form=o.COST_GRAM, **kw)
+        say("problem", p, "stops at", stg[p]["ao_iter"], "oracle", sto["ao_iter"])
+        assert 1 < sto["ao_iter"] < 120
+        assert logg[p][-1]["stopped"] == 1 and logg[p][-1]["accel_tried"] == 0
+        _check_problem(p, Fg[p], stg[p], logg[p], Fo, sto, logo, 1e-10, 1e-9)
+
+
+def test_batch_no_accel_no_norm(h):
+    probs = _problems(4, seed=13)[:8]
+    Xs = _tensors(probs, seed0=70)
+    init = [o.gen_factors(list(d), R, seed=300 + p) for p, (d, R) in enumerate(probs)]
+    h.batch_set(Xs, [R for _, R in probs])
+    kw = dict(max_iter=6, threshold_error=0.0, acceleration=0, normalization=0)
+    Fg, stg, logg, _ = h.cpd_batch(init, block=512, grid=4, **kw)
+    for p, ((dims, R), X) in enumerate(zip(probs, Xs)):
+        Fo, sto, logo = o.cpd(X, R, init[p], cost_form=o.COST_GRAM, **kw)
+        _check_problem(p, Fg[p], stg[p], logg[p], Fo, sto, logo, 1e-10, 1e-9)
+
+
+def test_batch_generator_and_init_match_oracle(h):
+    probs = [((20, 30, 40), 5), ((128, 7, 64), 16), ((3, 128, 2), 1)]
+    dims = [d for d, _ in probs]
+    ranks = [R for _, R in probs]
+    h.batch_generate(dims, ranks, seed=40, snr_db=25.0)
+    for p, (d, R) in enumerate(probs):
+        X, _ = o.make_tensor(list(d), R, seed=40 + p, snr_db=25.0)
+        X32 = X.astype(np.float32).astype(np.float64)
+        got = h.batch_tensor(p)
+        assert P.rel_err(got, X32) < 1e-7
+        assert np.mean(got == X32) > 0.999
+    F = h.batch_init_factors(seed=7)
+    for p, (d, R) in enumerate(probs):
+        ref = o.gen_factors(list(d), R, seed=7 + p)
+        for a, b in zip(F[p], ref):
+            assert np.array_equal(a, b)
+
+
+def test_batch_invalid_arguments(h):
+    X = [np.ones((4, 5, 6), order="F")]
+    with pytest.raises(P.PtnError):
+        h.batch_set([np.ones((129, 2, 2), order="F")], [2])       # mode above 128
+    with pytest.raises(P.PtnError):
+        h.batch_set(X, [17])                                       # rank above 16
+    h.batch_set(X, [3])
+    init = [[np.ones((I, 3)) for I in (4, 5, 6)]]
+    with pytest.raises(P.PtnError):
+        h.cpd_batch(init, block=128, grid=4)                       # unsupported block size
+    with pytest.raises(P.PtnError):
+        h.cpd_batch(init, block=256, grid=0)                       # grid missing
+    with pytest.raises(P.PtnError):
+        h.cpd_batch(init, constraints=[0, 1, 1])                   # unconstrained mode
