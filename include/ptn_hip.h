@@ -61,6 +61,11 @@ typedef struct {
   int      arith;               /* PTN_ARITH_F64       */
   int      max_inner;           /* Nesterov cap, NesterovMNLS.hpp:187: 50 (<= 255) */
   int      fuse_accel;          /* dimension tree + FP32: fold the acceleration pass into the next left-node pass */
+  /* the learned NNLS warm start (no reference counterpart) */
+  int      warm_start;          /* 1: start each NNLS from the handle's network where it passes the KKT test */
+  double   warm_kkt_tol;        /* a row takes the network's start if its relative KKT residual is <= this
+                                   or below the previous row's (default 1e-6) */
+  int      mlp_harvest;         /* 1: append every NNLS (inputs, solution) pair to the handle's training set */
 } ptn_options;
 
 /* Status<Tensor> (PARTENSOR_basic.hpp:556-595) minus the factors, which are written
@@ -84,6 +89,7 @@ typedef struct {
   double lipschitz[PTN_MAX_ORDER];
   double mu[PTN_MAX_ORDER];
   double t_end;        /* seconds from the start of the outer loop to the end of this sweep (device-synchronised) */
+  int    warm_rows[PTN_MAX_ORDER];  /* rows whose NNLS started from the network's proposal */
 } ptn_sweep_log;
 
 /* ---- handle ---- */
@@ -185,6 +191,38 @@ ptn_status ptn_batch_get_tensor(ptn_handle h, int p, double *X_out);
    rep: NULL or the shape used and the times. */
 ptn_status ptn_cpd_batch(ptn_handle h, const double *init_packed, const ptn_options *opt, double *out_packed,
                          ptn_result *res, ptn_batch_log *log, int block, int grid, ptn_batch_report *rep);
+
+/* ---- the learned NNLS warm start (no counterpart in the reference, whose NesterovMNLS
+        always starts from the previous factor).  A per-handle MLP, 2208 -> 256 -> 256 -> 64
+        with ReLU, BF16 weights, FP32 accumulation: input = the upper triangle of the R x R
+        Gram V (zero-padded to 64 x 64) scaled by its largest diagonal entry, the MTTKRP row
+        and the previous factor row (scaled); output = the scaled NNLS solution row.  The
+        network only proposes a start: the subproblem (its proximal anchor = the previous
+        row) and the stopping test stay the reference's, and rows whose proposal fails the
+        KKT test start from the previous row as in the reference ---- */
+#define PTN_MLP_IN 2208
+#define PTN_MLP_OUT 64
+#define PTN_MLP_PARAMS 647744   /* W1 256x2208, b1 256, W2 256x256, b2 256, W3 64x256, b3 64 (row-major) */
+
+/* Weights drawn He-uniform from seed (biases zero), Adam state cleared. */
+ptn_status ptn_mlp_init(ptn_handle h, uint64_t seed);
+ptn_status ptn_mlp_set_params(ptn_handle h, const float *params);
+ptn_status ptn_mlp_get_params(ptn_handle h, float *params);
+/* Y [n x 64] = network(X [n x 2208]) (row-major FP32 on the host, rounded to BF16). */
+ptn_status ptn_mlp_forward(ptn_handle h, int n, const float *X, float *Y);
+/* Mean squared error over the entries c < rank[i] of each row, and its gradient with
+   respect to every parameter (the layout of ptn_mlp_get_params). */
+ptn_status ptn_mlp_loss_grad(ptn_handle h, int n, const float *X, const float *T, const unsigned char *rank,
+                             double *loss, float *grad);
+/* The training set: pairs appended by ptn_cpd with mlp_harvest = 1, or given here. */
+ptn_status ptn_mlp_data_clear(ptn_handle h);
+ptn_status ptn_mlp_data_add(ptn_handle h, int n, const float *X, const float *T, const unsigned char *rank);
+ptn_status ptn_mlp_data_size(ptn_handle h, int64_t *n);
+/* Rows [first, first + n) of the training set: features (BF16 values as FP32), targets, ranks. */
+ptn_status ptn_mlp_data_get(ptn_handle h, int64_t first, int n, float *X, float *T, unsigned char *rank);
+/* Adam (lr, beta 0.9 / 0.999) on minibatches of `batch` rows (a multiple of 32) drawn
+   without replacement (seeded shuffles) for `epochs` epochs; epoch_loss[e] = mean batch loss. */
+ptn_status ptn_mlp_train(ptn_handle h, int epochs, int batch, double lr, uint64_t seed, double *epoch_loss);
 
 #ifdef __cplusplus
 }

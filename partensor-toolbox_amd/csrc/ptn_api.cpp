@@ -15,6 +15,7 @@
 #include <memory>
 #include <new>
 #include <numeric>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -32,6 +33,10 @@ struct BatchSet {
   }
 };
 
+namespace {
+struct MlpState;
+}
+
 struct ptn_handle_s {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -48,6 +53,7 @@ struct ptn_handle_s {
   bool frob_valid = false;
   double frob_host = 0.0;
   BatchSet *batch = nullptr;
+  MlpState *mlp = nullptr;    // the warm-start network, its training set and work buffers
 };
 
 namespace {
@@ -125,6 +131,110 @@ struct EventPair {                           // two timing-free events, destroye
 };
 
 ptn_status run(ptn_handle h, const std::function<void()> &fn);
+
+// ------------------------------------------------------------------------------------
+// the warm-start network: parameters, Adam state, training set, activations
+// ------------------------------------------------------------------------------------
+uint16_t f32_to_bf16(float f) {               // round to nearest even; NaN stays NaN
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+struct MlpState {
+  ptn_handle h;
+  bool ready = false;
+  DevBuf<float> w, g, m1, m2;                 // FP32 master parameters, gradient, Adam moments
+  DevBuf<uint16_t> wb, w2t, w3t;              // bf16 W1 | W2 | W3, and W2^T, W3^T
+  int step = 0;
+  DevBuf<uint16_t> dx;                        // training set: features (bf16), targets, ranks
+  DevBuf<float> dt;
+  DevBuf<unsigned char> drank;
+  std::vector<unsigned char> hrank;
+  int64_t n = 0, cap = 0;
+  int rows = 0;                               // activation buffers hold this many rows
+  DevBuf<uint16_t> X, XT, H1, H2, H1T, H2T, dYb, dYT, dH1, dH1T, dH2, dH2T;
+  DevBuf<float> Y, T, dYf, scale;
+  DevBuf<unsigned char> rank;
+  DevBuf<double> start, part;
+  explicit MlpState(ptn_handle hh) : h(hh) {
+    w.alloc(h, kMlpParams); g.alloc(h, kMlpParams); m1.alloc(h, kMlpParams); m2.alloc(h, kMlpParams);
+    wb.alloc(h, kMlpWb); w2t.alloc(h, (size_t)kMlpH * kMlpH); w3t.alloc(h, (size_t)kMlpH * kMlpR);
+    part.alloc(h, 64);
+  }
+  template <typename T_>
+  static void grow(ptn_handle h, DevBuf<T_> &b, size_t count) {
+    if (b.p && b.n >= count) return;
+    b.~DevBuf();
+    new (&b) DevBuf<T_>();
+    b.alloc(h, count);
+  }
+  void ensure_rows(int r) {
+    r = (r + 31) / 32 * 32;
+    if (r <= rows) return;
+    HIPCHK(hipDeviceSynchronize());
+    const size_t rin = (size_t)r * kMlpIn, rh = (size_t)r * kMlpH, ro = (size_t)r * kMlpR;
+    grow(h, X, rin); grow(h, XT, rin); grow(h, H1, rh); grow(h, H2, rh); grow(h, H1T, rh); grow(h, H2T, rh);
+    grow(h, dYb, ro); grow(h, dYT, ro); grow(h, dH1, rh); grow(h, dH1T, rh); grow(h, dH2, rh); grow(h, dH2T, rh);
+    grow(h, Y, ro); grow(h, T, ro); grow(h, dYf, ro); grow(h, scale, (size_t)r); grow(h, rank, (size_t)r);
+    grow(h, start, ro);
+    rows = r;
+  }
+  void ensure_data(int64_t need) {
+    if (need <= cap) return;
+    int64_t nc = std::max<int64_t>(need, 2 * cap);
+    HIPCHK(hipDeviceSynchronize());
+    DevBuf<uint16_t> nx(h, (size_t)nc * kMlpIn);
+    DevBuf<float> nt(h, (size_t)nc * kMlpR);
+    DevBuf<unsigned char> nr(h, (size_t)nc);
+    if (n > 0) {
+      HIPCHK(hipMemcpy(nx.p, dx.p, (size_t)n * kMlpIn * 2, hipMemcpyDeviceToDevice));
+      HIPCHK(hipMemcpy(nt.p, dt.p, (size_t)n * kMlpR * 4, hipMemcpyDeviceToDevice));
+      HIPCHK(hipMemcpy(nr.p, drank.p, (size_t)n, hipMemcpyDeviceToDevice));
+    }
+    std::swap(dx.p, nx.p); std::swap(dx.n, nx.n); dx.h = h;
+    std::swap(dt.p, nt.p); std::swap(dt.n, nt.n); dt.h = h;
+    std::swap(drank.p, nr.p); std::swap(drank.n, nr.n); drank.h = h;
+    cap = nc;
+  }
+  // Y = network(X) for B rows (activations kept for the backward pass)
+  void forward(hipStream_t s, int B) {
+    gemm_nt(s, B, kMlpH, kMlpIn, X.p, kMlpIn, wb.p + kMlpW1b, kMlpIn, w.p + kMlpB1, nullptr, 0, 1, nullptr, 0, H1.p,
+            kMlpH);
+    gemm_nt(s, B, kMlpH, kMlpH, H1.p, kMlpH, wb.p + kMlpW2b, kMlpH, w.p + kMlpB2, nullptr, 0, 1, nullptr, 0, H2.p,
+            kMlpH);
+    gemm_nt(s, B, kMlpR, kMlpH, H2.p, kMlpH, wb.p + kMlpW3b, kMlpH, w.p + kMlpB3, nullptr, 0, 0, Y.p, kMlpR, nullptr,
+            0);
+  }
+  // gradient of the masked mean squared error into g (B a multiple of 32); per-block
+  // partial losses in part[0..64)
+  void backward(hipStream_t s, int B, float inv_count) {
+    mse_grad(s, B, Y.p, T.p, rank.p, inv_count, dYb.p, dYf.p, part.p, 64);
+    colsum_f32(s, B, kMlpR, dYf.p, kMlpR, g.p + kMlpB3);
+    transpose_bf16(s, B, kMlpR, dYb.p, kMlpR, dYT.p, B);
+    transpose_bf16(s, B, kMlpH, H2.p, kMlpH, H2T.p, B);
+    gemm_nt(s, kMlpR, kMlpH, B, dYT.p, B, H2T.p, B, nullptr, nullptr, 0, 0, g.p + kMlpW3, kMlpH, nullptr, 0);
+    transpose_bf16(s, kMlpR, kMlpH, wb.p + kMlpW3b, kMlpH, w3t.p, kMlpR);
+    gemm_nt(s, B, kMlpH, kMlpR, dYb.p, kMlpR, w3t.p, kMlpR, nullptr, H2.p, kMlpH, 0, nullptr, 0, dH2.p, kMlpH);
+    colsum_bf16(s, B, kMlpH, dH2.p, kMlpH, g.p + kMlpB2);
+    transpose_bf16(s, B, kMlpH, dH2.p, kMlpH, dH2T.p, B);
+    transpose_bf16(s, B, kMlpH, H1.p, kMlpH, H1T.p, B);
+    gemm_nt(s, kMlpH, kMlpH, B, dH2T.p, B, H1T.p, B, nullptr, nullptr, 0, 0, g.p + kMlpW2, kMlpH, nullptr, 0);
+    transpose_bf16(s, kMlpH, kMlpH, wb.p + kMlpW2b, kMlpH, w2t.p, kMlpH);
+    gemm_nt(s, B, kMlpH, kMlpH, dH2.p, kMlpH, w2t.p, kMlpH, nullptr, H1.p, kMlpH, 0, nullptr, 0, dH1.p, kMlpH);
+    colsum_bf16(s, B, kMlpH, dH1.p, kMlpH, g.p + kMlpB1);
+    transpose_bf16(s, B, kMlpH, dH1.p, kMlpH, dH1T.p, B);
+    transpose_bf16(s, B, kMlpIn, X.p, kMlpIn, XT.p, B);
+    gemm_nt(s, kMlpH, kMlpIn, B, dH1T.p, B, XT.p, B, nullptr, nullptr, 0, 0, g.p + kMlpW1, kMlpIn, nullptr, 0);
+  }
+};
+
+MlpState &mlp_of(ptn_handle h) {
+  if (!h->mlp) h->mlp = new MlpState(h);
+  return *h->mlp;
+}
 
 int round_up16(int R) { return (R + 15) / 16 * 16; }
 
@@ -556,11 +666,30 @@ void update_nonneg(Work &w, int mode, const ptn_options &o, const ModeIn &in) {
     w.hist.alloc(h, need);
     w.hist_cap = need;
   }
-  // the start point A_old stays in hist[0] (written by the iterate kernel) for the fallback
-  nnls_iterate(s, I, w.R, w.ldr, in.V, p, in.betas, w.M, A, o.nesterov_delta_1, o.nesterov_delta_2, o.max_inner,
-               w.mask, w.hist, w.colsum);
+  const bool warm = o.warm_start != 0, harvest = o.mlp_harvest != 0;
+  MlpState *ms = (warm || harvest) ? &mlp_of(h) : nullptr;
+  if (warm || harvest) HIPCHK(hipMemcpyAsync(w.Aold, A, I * w.ldr * sizeof(double), hipMemcpyDeviceToDevice, s));
+  const double *start = nullptr;
+  if (warm) {
+    // the network proposes each row's start; rows failing the KKT test keep A (k_warm_select)
+    mlp_features(s, I, w.R, w.ldr, in.V, w.M, A, nullptr, ms->X.p, nullptr, nullptr, ms->scale.p);
+    ms->forward(s, (int)I);
+    warm_select(s, I, w.R, w.ldr, in.V, p, w.M, A, ms->Y.p, ms->scale.p, o.warm_kkt_tol, ms->start.p,
+                &p->warm_rows);
+    start = ms->start.p;
+  }
+  // without a warm start the start point A_old stays in hist[0] for the fallback
+  nnls_iterate(s, I, w.R, w.ldr, in.V, p, in.betas, w.M, A, start, o.nesterov_delta_1, o.nesterov_delta_2,
+               o.max_inner, w.mask, w.hist, w.colsum);
   nnls_select(s, I, w.R, w.ldr, w.hist, w.mask, o.max_inner, A, w.colsum, p);
-  zero_col_fallback(s, I, w.R, w.ldr, w.colsum, w.hist, A, p);
+  zero_col_fallback(s, I, w.R, w.ldr, w.colsum, warm ? w.Aold.p : w.hist.p, A, p);
+  if (harvest) {
+    // (V, M row, previous row) -> the solution row, appended to the training set
+    mlp_features(s, I, w.R, w.ldr, in.V, w.M, w.Aold, A, ms->dx.p + ms->n * kMlpIn, ms->dt.p + ms->n * kMlpR,
+                 ms->drank.p + ms->n, nullptr);
+    ms->n += I;
+    ms->hrank.insert(ms->hrank.end(), (size_t)I, (unsigned char)w.R);
+  }
 }
 
 }  // namespace
@@ -586,6 +715,9 @@ void ptn_default_options(ptn_options *o) {
   o->arith = PTN_ARITH_F64;
   o->max_inner = 50;
   o->fuse_accel = 0;
+  o->warm_start = 0;
+  o->warm_kkt_tol = 1e-6;
+  o->mlp_harvest = 0;
 }
 
 ptn_status ptn_create(ptn_handle *out, int device) {
@@ -620,6 +752,7 @@ void ptn_destroy(ptn_handle h) {
   if (h->x64) (void)hipFree(h->x64);
   if (h->x32) (void)hipFree(h->x32);
   delete h->batch;
+  delete h->mlp;
   if (h->scal) (void)hipFree(h->scal);
   if (h->part) (void)hipFree(h->part);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -771,8 +904,19 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
     const bool pair = o.fuse_accel != 0;
     REQUIRE(!pair || (tree && (o.arith == PTN_ARITH_F32 || o.arith == PTN_ARITH_BF16X3)), PTN_ERR_UNSUPPORTED,
             "fuse_accel needs the dimension tree with F32 or BF16X3 contractions");
+    REQUIRE(!o.warm_start || (h->mlp && h->mlp->ready), PTN_ERR_STATE,
+            "warm_start needs a network (ptn_mlp_init or ptn_mlp_set_params)");
 
     Work w(h, R, o.arith, tree);
+    if (o.warm_start || o.mlp_harvest) {
+      MlpState &mst = mlp_of(h);
+      mst.ensure_rows((int)w.maxI);
+      if (o.mlp_harvest) {
+        int64_t per = 0;
+        for (int k = 0; k < w.N; ++k) per += w.dims[k];
+        mst.ensure_data(mst.n + per * o.max_iter);
+      }
+    }
     if (pair && o.acceleration) w.enable_pair();
     const int N = w.N, last = N - 1;
     upload_factors(w, init_packed, w.fac);
@@ -934,6 +1078,7 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
         lg.zero_col_fallback[n] = rb->hp[n].fallback;
         lg.lipschitz[n] = rb->hp[n].L0;
         lg.mu[n] = rb->hp[n].mu0;
+        lg.warm_rows[n] = rb->hp[n].warm_rows;
       }
       const bool stop = r.rel_cost < o.threshold_error || it >= (int)o.max_iter;
       if (tried_at[it & 1] && !stop) {
@@ -1049,8 +1194,8 @@ ptn_status ptn_nesterov_mnls(ptn_handle h, int64_t I, int R, const double *Z, co
     colmajor_to_rows(h->stream, I, R, ldr, tmp.p, dA.p);
     eig_extremes(h->stream, R, dZ.p, p.p);
     nnls_prep(h->stream, p.p, max_inner, betas.p);
-    nnls_iterate(h->stream, I, R, ldr, dZ.p, p.p, betas.p, dW.p, dA.p, delta_1, delta_2, max_inner, mask.p, hist.p,
-                 colsum.p);
+    nnls_iterate(h->stream, I, R, ldr, dZ.p, p.p, betas.p, dW.p, dA.p, nullptr, delta_1, delta_2, max_inner, mask.p,
+                 hist.p, colsum.p);
     nnls_select(h->stream, I, R, ldr, hist.p, mask.p, max_inner, dB.p, colsum.p, p.p);
     rows_to_colmajor(h->stream, I, R, ldr, dB.p, tmp.p);
     HIPCHK(hipMemcpyAsync(A, tmp.p, I * R * sizeof(double), hipMemcpyDeviceToHost, h->stream));
@@ -1546,6 +1691,213 @@ ptn_status ptn_cpd_batch(ptn_handle h, const double *init_packed, const ptn_opti
       rep->tune_seconds = tune_s;
       rep->sweeps = o.max_iter;
     }
+  });
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------
+// the warm-start network
+// ------------------------------------------------------------------------------------
+namespace {
+
+// n host rows (FP32) into the activation buffers as bf16, zero rows up to the padded count
+void upload_rows(MlpState &ms, int n, int np, const float *X, const float *T, const unsigned char *rank) {
+  ptn_handle h = ms.h;
+  std::vector<uint16_t> xb((size_t)np * kMlpIn, 0);
+  for (size_t e = 0; e < (size_t)n * kMlpIn; ++e) xb[e] = f32_to_bf16(X[e]);
+  HIPCHK(hipMemcpy(ms.X.p, xb.data(), xb.size() * 2, hipMemcpyHostToDevice));
+  if (T) {
+    std::vector<float> t((size_t)np * kMlpR, 0.f);
+    std::memcpy(t.data(), T, (size_t)n * kMlpR * sizeof(float));
+    HIPCHK(hipMemcpy(ms.T.p, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));
+    std::vector<unsigned char> r((size_t)np, 0);
+    std::memcpy(r.data(), rank, (size_t)n);
+    HIPCHK(hipMemcpy(ms.rank.p, r.data(), r.size(), hipMemcpyHostToDevice));
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+ptn_status ptn_mlp_init(ptn_handle h, uint64_t seed) {
+  return run(h, [&] {
+    MlpState &ms = mlp_of(h);
+    std::vector<float> w(kMlpParams, 0.f);
+    std::mt19937_64 rng(seed);
+    auto fill = [&](int64_t off, int64_t count, int fan_in) {
+      std::uniform_real_distribution<double> u(-std::sqrt(6.0 / fan_in), std::sqrt(6.0 / fan_in));
+      for (int64_t e = 0; e < count; ++e) w[off + e] = (float)u(rng);
+    };
+    fill(kMlpW1, (int64_t)kMlpH * kMlpIn, kMlpIn);
+    fill(kMlpW2, (int64_t)kMlpH * kMlpH, kMlpH);
+    fill(kMlpW3, (int64_t)kMlpR * kMlpH, kMlpH);
+    HIPCHK(hipMemcpy(ms.w.p, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(ms.m1.p, 0, kMlpParams * sizeof(float)));
+    HIPCHK(hipMemset(ms.m2.p, 0, kMlpParams * sizeof(float)));
+    weights_bf16(h->stream, kMlpParams, ms.w.p, ms.wb.p);
+    HIPCHK(hipStreamSynchronize(h->stream));
+    ms.step = 0;
+    ms.ready = true;
+  });
+}
+
+ptn_status ptn_mlp_set_params(ptn_handle h, const float *params) {
+  return run(h, [&] {
+    REQUIRE(params, PTN_ERR_INVALID, "params is null");
+    MlpState &ms = mlp_of(h);
+    HIPCHK(hipMemcpy(ms.w.p, params, kMlpParams * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(ms.m1.p, 0, kMlpParams * sizeof(float)));
+    HIPCHK(hipMemset(ms.m2.p, 0, kMlpParams * sizeof(float)));
+    weights_bf16(h->stream, kMlpParams, ms.w.p, ms.wb.p);
+    HIPCHK(hipStreamSynchronize(h->stream));
+    ms.step = 0;
+    ms.ready = true;
+  });
+}
+
+ptn_status ptn_mlp_get_params(ptn_handle h, float *params) {
+  return run(h, [&] {
+    REQUIRE(params, PTN_ERR_INVALID, "params is null");
+    REQUIRE(h->mlp && h->mlp->ready, PTN_ERR_STATE, "no network on the handle");
+    HIPCHK(hipMemcpy(params, h->mlp->w.p, kMlpParams * sizeof(float), hipMemcpyDeviceToHost));
+  });
+}
+
+ptn_status ptn_mlp_forward(ptn_handle h, int n, const float *X, float *Y) {
+  return run(h, [&] {
+    REQUIRE(X && Y, PTN_ERR_INVALID, "null argument");
+    REQUIRE(n >= 1 && n <= (1 << 22), PTN_ERR_INVALID, "n must be in [1, 2^22]");
+    REQUIRE(h->mlp && h->mlp->ready, PTN_ERR_STATE, "no network on the handle");
+    MlpState &ms = *h->mlp;
+    ms.ensure_rows(n);
+    upload_rows(ms, n, n, X, nullptr, nullptr);
+    ms.forward(h->stream, n);
+    HIPCHK(hipMemcpyAsync(Y, ms.Y.p, (size_t)n * kMlpR * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipGetLastError());
+  });
+}
+
+ptn_status ptn_mlp_loss_grad(ptn_handle h, int n, const float *X, const float *T, const unsigned char *rank,
+                             double *loss, float *grad) {
+  return run(h, [&] {
+    REQUIRE(X && T && rank && loss && grad, PTN_ERR_INVALID, "null argument");
+    REQUIRE(n >= 1 && n <= (1 << 20), PTN_ERR_INVALID, "n must be in [1, 2^20]");
+    REQUIRE(h->mlp && h->mlp->ready, PTN_ERR_STATE, "no network on the handle");
+    int64_t count = 0;
+    for (int i = 0; i < n; ++i) {
+      REQUIRE(rank[i] <= kMlpR, PTN_ERR_INVALID, "rank above 64");
+      count += rank[i];
+    }
+    REQUIRE(count > 0, PTN_ERR_INVALID, "no valid entries");
+    MlpState &ms = *h->mlp;
+    const int np = (n + 31) / 32 * 32;
+    ms.ensure_rows(np);
+    upload_rows(ms, n, np, X, T, rank);
+    ms.forward(h->stream, np);
+    ms.backward(h->stream, np, (float)(1.0 / (double)count));
+    std::vector<double> part(64);
+    HIPCHK(hipMemcpyAsync(part.data(), ms.part.p, 64 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(grad, ms.g.p, kMlpParams * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipGetLastError());
+    double sum = 0.0;
+    for (double v : part) sum += v;
+    *loss = sum;
+  });
+}
+
+ptn_status ptn_mlp_data_clear(ptn_handle h) {
+  return run(h, [&] {
+    MlpState &ms = mlp_of(h);
+    ms.n = 0;
+    ms.hrank.clear();
+  });
+}
+
+ptn_status ptn_mlp_data_add(ptn_handle h, int n, const float *X, const float *T, const unsigned char *rank) {
+  return run(h, [&] {
+    REQUIRE(X && T && rank, PTN_ERR_INVALID, "null argument");
+    REQUIRE(n >= 1 && n <= (1 << 24), PTN_ERR_INVALID, "n must be in [1, 2^24]");
+    for (int i = 0; i < n; ++i) REQUIRE(rank[i] <= kMlpR, PTN_ERR_INVALID, "rank above 64");
+    MlpState &ms = mlp_of(h);
+    ms.ensure_data(ms.n + n);
+    std::vector<uint16_t> xb((size_t)n * kMlpIn);
+    for (size_t e = 0; e < xb.size(); ++e) xb[e] = f32_to_bf16(X[e]);
+    HIPCHK(hipMemcpy(ms.dx.p + ms.n * kMlpIn, xb.data(), xb.size() * 2, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ms.dt.p + ms.n * kMlpR, T, (size_t)n * kMlpR * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ms.drank.p + ms.n, rank, (size_t)n, hipMemcpyHostToDevice));
+    ms.hrank.insert(ms.hrank.end(), rank, rank + n);
+    ms.n += n;
+  });
+}
+
+ptn_status ptn_mlp_data_size(ptn_handle h, int64_t *n) {
+  return run(h, [&] {
+    REQUIRE(n, PTN_ERR_INVALID, "null argument");
+    *n = h->mlp ? h->mlp->n : 0;
+  });
+}
+
+ptn_status ptn_mlp_data_get(ptn_handle h, int64_t first, int n, float *X, float *T, unsigned char *rank) {
+  return run(h, [&] {
+    REQUIRE(X && T && rank, PTN_ERR_INVALID, "null argument");
+    REQUIRE(h->mlp && first >= 0 && n >= 0 && first + n <= h->mlp->n, PTN_ERR_INVALID, "rows out of range");
+    MlpState &ms = *h->mlp;
+    std::vector<uint16_t> xb((size_t)n * kMlpIn);
+    HIPCHK(hipMemcpy(xb.data(), ms.dx.p + first * kMlpIn, xb.size() * 2, hipMemcpyDeviceToHost));
+    for (size_t e = 0; e < xb.size(); ++e) {
+      const uint32_t u = (uint32_t)xb[e] << 16;
+      std::memcpy(&X[e], &u, 4);
+    }
+    HIPCHK(hipMemcpy(T, ms.dt.p + first * kMlpR, (size_t)n * kMlpR * sizeof(float), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(rank, ms.drank.p + first, (size_t)n, hipMemcpyDeviceToHost));
+  });
+}
+
+ptn_status ptn_mlp_train(ptn_handle h, int epochs, int batch, double lr, uint64_t seed, double *epoch_loss) {
+  return run(h, [&] {
+    REQUIRE(h->mlp && h->mlp->ready, PTN_ERR_STATE, "no network on the handle");
+    MlpState &ms = *h->mlp;
+    REQUIRE(epochs >= 1 && epochs <= 100000, PTN_ERR_INVALID, "epochs must be in [1, 1e5]");
+    REQUIRE(batch >= 32 && batch % 32 == 0 && batch <= 65536, PTN_ERR_INVALID,
+            "batch must be a multiple of 32 in [32, 65536]");
+    REQUIRE(ms.n >= batch, PTN_ERR_STATE, "the training set holds fewer rows than one batch");
+    REQUIRE(lr > 0.0, PTN_ERR_INVALID, "lr must be positive");
+    hipStream_t s = h->stream;
+    ms.ensure_rows(batch);
+    const int64_t spe = ms.n / batch;
+    DevBuf<int> idx(h, (size_t)ms.n);
+    DevBuf<double> losses(h, (size_t)(spe * epochs));
+    std::vector<int> perm((size_t)ms.n);
+    std::mt19937_64 rng(seed);
+    for (int ep = 0; ep < epochs; ++ep) {
+      std::iota(perm.begin(), perm.end(), 0);
+      std::shuffle(perm.begin(), perm.end(), rng);
+      HIPCHK(hipMemcpyAsync(idx.p, perm.data(), perm.size() * sizeof(int), hipMemcpyHostToDevice, s));
+      for (int64_t st = 0; st < spe; ++st) {
+        int64_t count = 0;
+        for (int64_t r = st * batch; r < (st + 1) * batch; ++r) count += ms.hrank[perm[r]];
+        gather_rows(s, batch, idx.p + st * batch, ms.dx.p, ms.dt.p, ms.drank.p, ms.X.p, ms.T.p, ms.rank.p);
+        ms.forward(s, batch);
+        ms.backward(s, batch, count > 0 ? (float)(1.0 / (double)count) : 0.f);
+        adam_step(s, kMlpParams, ms.w.p, ms.g.p, ms.m1.p, ms.m2.p, (float)lr, 0.9f, 0.999f, 1e-8f, ++ms.step, ms.wb.p);
+        sum_partials(s, ms.part.p, 64, losses.p + ep * spe + st);
+      }
+      // the host permutation buffer is rewritten next epoch: wait for the upload to land
+      HIPCHK(hipStreamSynchronize(s));
+    }
+    std::vector<double> l((size_t)(spe * epochs));
+    HIPCHK(hipMemcpy(l.data(), losses.p, l.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipGetLastError());
+    if (epoch_loss)
+      for (int ep = 0; ep < epochs; ++ep) {
+        double sm = 0.0;
+        for (int64_t st = 0; st < spe; ++st) sm += l[ep * spe + st];
+        epoch_loss[ep] = sm / (double)spe;
+      }
   });
 }
 

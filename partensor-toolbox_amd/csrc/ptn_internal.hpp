@@ -32,6 +32,8 @@ struct NnlsParams {
   double L, lambda, q;  // after GLambda
   int iters;            // projected-gradient steps taken (k*)
   int fallback;         // zero-column blend applied
+  int warm_rows;        // rows started from the warm-start network's proposal
+  int pad;
 };
 
 // ---- batched small CPDs (batch_kernels.hip) ----
@@ -82,6 +84,39 @@ size_t batch_lds_bytes();
 void batch_init(hipStream_t s, const BatchArgs &a);
 void batch_sweep(hipStream_t s, const BatchArgs &a, int block, int grid);
 void batch_next_sweep(hipStream_t s, int *ctr);
+
+// ---- the NNLS warm-start network (mlp_kernels.hip) ----
+// 2208 -> 256 -> 256 -> 64 with ReLU: inputs vech(V) (64 x 65 / 2) + MTTKRP row + previous row.
+constexpr int kMlpR = 64, kMlpVech = kMlpR * (kMlpR + 1) / 2, kMlpIn = kMlpVech + 2 * kMlpR, kMlpH = 256;
+// FP32 master vector: W1 [H x In], b1 [H], W2 [H x H], b2 [H], W3 [R x H], b3 [R]
+constexpr int64_t kMlpW1 = 0, kMlpB1 = kMlpW1 + (int64_t)kMlpH * kMlpIn, kMlpW2 = kMlpB1 + kMlpH,
+                  kMlpB2 = kMlpW2 + (int64_t)kMlpH * kMlpH, kMlpW3 = kMlpB2 + kMlpH,
+                  kMlpB3 = kMlpW3 + (int64_t)kMlpR * kMlpH, kMlpParams = kMlpB3 + kMlpR;
+// bf16 copies of W1, W2, W3 back to back
+constexpr int64_t kMlpW1b = 0, kMlpW2b = (int64_t)kMlpH * kMlpIn, kMlpW3b = kMlpW2b + (int64_t)kMlpH * kMlpH,
+                  kMlpWb = kMlpW3b + (int64_t)kMlpR * kMlpH;
+__host__ __device__ inline int64_t mlp_bf16_index(int64_t e) {
+  if (e < kMlpB1) return e;
+  if (e >= kMlpW2 && e < kMlpB2) return e - kMlpW2 + kMlpW2b;
+  if (e >= kMlpW3 && e < kMlpB3) return e - kMlpW3 + kMlpW3b;
+  return -1;
+}
+void gemm_nt(hipStream_t s, int M, int N, int K, const void *A, int lda, const void *B, int ldb, const float *bias,
+             const void *gate, int ldg, int relu, float *C, int ldc, void *Cb, int ldcb);
+void transpose_bf16(hipStream_t s, int rows, int cols, const void *in, int ldi, void *out, int ldo);
+void mlp_features(hipStream_t s, int64_t I, int R, int ldr, const double *V, const double *M, const double *A,
+                  const double *Anew, void *X, float *T, unsigned char *rank, float *scale);
+void warm_select(hipStream_t s, int64_t I, int R, int ldr, const double *V, const NnlsParams *p, const double *M,
+                 const double *A, const float *Y, const float *scale, double kkt_tol, double *S, int *count);
+void mse_grad(hipStream_t s, int B, const float *Y, const float *T, const unsigned char *rank, float inv_count,
+              void *dYb, float *dY, double *part, int nparts);
+void colsum_f32(hipStream_t s, int M, int N, const float *G, int ldg, float *out);
+void colsum_bf16(hipStream_t s, int M, int N, const void *G, int ldg, float *out);
+void adam_step(hipStream_t s, int64_t n, float *w, const float *g, float *m1, float *m2, float lr, float b1, float b2,
+               float eps, int step, void *wb);
+void weights_bf16(hipStream_t s, int64_t n, const float *w, void *wb);
+void gather_rows(hipStream_t s, int B, const int *idx, const void *X, const float *T, const unsigned char *rank,
+                 void *Xb, float *Tb, unsigned char *rb);
 
 // ---- generation (gen_kernels.hip) ----
 void gen_factors(hipStream_t s, int N, const int64_t *dims, int R, uint64_t seed, int dist, double *packed);
@@ -154,9 +189,11 @@ void nnls_prep(hipStream_t s, const NnlsParams *p, int max_inner, double *betas)
 // satisfaction masks of nnls_iterate: 4 words per block of 4 rows
 inline int64_t nnls_mask_words(int64_t I) { return (I + 3) / 4 * 4; }
 // hist: (max_inner + 1) x I x ldr doubles; mask: nnls_mask_words(I) words
+// A: the previous factor rows (the proximal anchor, and the start when S is null);
+// S: another start (the warm start)
 void nnls_iterate(hipStream_t s, int64_t I, int R, int ldr, const double *V, const NnlsParams *p,
-                  const double *betas, const double *M, const double *A, double d1, double d2, int max_inner,
-                  uint64_t *mask, double *hist, double *colsum);   // zeroes colsum for nnls_select
+                  const double *betas, const double *M, const double *A, const double *S, double d1, double d2,
+                  int max_inner, uint64_t *mask, double *hist, double *colsum);   // zeroes colsum for nnls_select
 void nnls_select(hipStream_t s, int64_t I, int R, int ldr, const double *hist, const uint64_t *mask, int max_inner,
                  double *Anew, double *colsum, NnlsParams *p);
 void zero_col_fallback(hipStream_t s, int64_t I, int R, int ldr, const double *colsum, const double *Aold,

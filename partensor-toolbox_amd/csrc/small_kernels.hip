@@ -133,7 +133,7 @@ __global__ __launch_bounds__(256) void k_nnls_iterate(int64_t I, int R, int ldr,
                                                       const NnlsParams *__restrict__ prm,
                                                       const double *__restrict__ betas,
                                                       const double *__restrict__ M, const double *__restrict__ A,
-                                                      double d1, double d2, int max_inner,
+                                                      const double *__restrict__ S, double d1, double d2, int max_inner,
                                                       unsigned long long *mask, double *__restrict__ hist,
                                                       double *colsum) {
   __shared__ double sbeta[kMaxInner + 1];
@@ -156,9 +156,11 @@ __global__ __launch_bounds__(256) void k_nnls_iterate(int64_t I, int R, int ldr,
     const double vk = (on && k < R) ? V[k * R + r] : 0.0;
     z[k] = (k == r && on) ? vk + lambda : vk;
   }
-  // lanes >= R: zero start, zero column of Zt and zero Wt keep every iterate at +0
-  const double a0 = on ? A[row * ldr + r] : 0.0;
-  const double wt = on ? M[row * ldr + r] + lambda * a0 : 0.0;
+  // lanes >= R: zero start, zero column of Zt and zero Wt keep every iterate at +0.
+  // The anchor of W~ is always the previous row A; the start is A or the warm start S.
+  const double anc = on ? A[row * ldr + r] : 0.0;
+  const double a0 = on ? (S ? S[row * ldr + r] : anc) : 0.0;
+  const double wt = on ? M[row * ldr + r] + lambda * anc : 0.0;
   double y = a0, acur = a0;
   const int64_t hstride = I * ldr;
   double *hrow = hist + row * ldr + lane;
@@ -465,12 +467,12 @@ void nnls_prep(hipStream_t s, const NnlsParams *p, int max_inner, double *betas)
 }
 
 void nnls_iterate(hipStream_t s, int64_t I, int R, int ldr, const double *V, const NnlsParams *p,
-                  const double *betas, const double *M, const double *A, double d1, double d2, int max_inner,
-                  uint64_t *mask, double *hist, double *colsum) {
+                  const double *betas, const double *M, const double *A, const double *S, double d1, double d2,
+                  int max_inner, uint64_t *mask, double *hist, double *colsum) {
   const unsigned grid = (unsigned)((I + 3) / 4);
   unsigned long long *m = (unsigned long long *)mask;
 #define PTN_NI(RBV) \
-  k_nnls_iterate<RBV><<<grid, 256, 0, s>>>(I, R, ldr, V, p, betas, M, A, d1, d2, max_inner, m, hist, colsum)
+  k_nnls_iterate<RBV><<<grid, 256, 0, s>>>(I, R, ldr, V, p, betas, M, A, S, d1, d2, max_inner, m, hist, colsum)
   if (R <= 16) PTN_NI(16);
   else if (R <= 32) PTN_NI(32);
   else PTN_NI(64);

@@ -30,7 +30,10 @@ EXPORTED = [
     "ptn_cpd", "ptn_mttkrp", "ptn_mttkrp_dimtree", "ptn_nesterov_mnls", "ptn_sym_extremes",
     "ptn_normalize", "ptn_cost_explicit", "ptn_time_contraction",
     "ptn_batch_set", "ptn_batch_generate", "ptn_batch_init_factors", "ptn_batch_get_tensor", "ptn_cpd_batch",
+    "ptn_mlp_init", "ptn_mlp_set_params", "ptn_mlp_get_params", "ptn_mlp_forward", "ptn_mlp_loss_grad",
+    "ptn_mlp_data_clear", "ptn_mlp_data_add", "ptn_mlp_data_size", "ptn_mlp_data_get", "ptn_mlp_train",
 ]
+MLP_IN, MLP_OUT, MLP_PARAMS = 2208, 64, 647744
 
 _d, _i, _i64, _u64 = ctypes.c_double, ctypes.c_int, ctypes.c_int64, ctypes.c_uint64
 _pd = ctypes.POINTER(ctypes.c_double)
@@ -44,7 +47,7 @@ class Options(ctypes.Structure):
         ("max_iter", _i), ("threshold_error", _d), ("nesterov_delta_1", _d), ("nesterov_delta_2", _d),
         ("accel_coeff", _i), ("accel_fail", _i), ("acceleration", _i), ("normalization", _i),
         ("constraints", _i * MAX_ORDER), ("mttkrp_form", _i), ("arith", _i), ("max_inner", _i),
-        ("fuse_accel", _i),
+        ("fuse_accel", _i), ("warm_start", _i), ("warm_kkt_tol", _d), ("mlp_harvest", _i),
     ]
 
 
@@ -57,7 +60,7 @@ class SweepLog(ctypes.Structure):
     _fields_ = [
         ("f_value", _d), ("rel_cost", _d), ("f_accel", _d), ("accel_tried", _i), ("accel_accepted", _i),
         ("nnls_iters", _i * MAX_ORDER), ("zero_col_fallback", _i * MAX_ORDER),
-        ("lipschitz", _d * MAX_ORDER), ("mu", _d * MAX_ORDER), ("t_end", _d),
+        ("lipschitz", _d * MAX_ORDER), ("mu", _d * MAX_ORDER), ("t_end", _d), ("warm_rows", _i * MAX_ORDER),
     ]
 
 
@@ -113,12 +116,32 @@ def lib():
         L.ptn_batch_get_tensor.argtypes = [_vp, _i, _pd]
         L.ptn_cpd_batch.argtypes = [_vp, _pd, ctypes.POINTER(Options), _pd, ctypes.POINTER(Result),
                                     ctypes.POINTER(BatchLog), _i, _i, ctypes.POINTER(BatchReport)]
+        _pf = ctypes.POINTER(ctypes.c_float)
+        _pu8 = ctypes.POINTER(ctypes.c_ubyte)
+        L.ptn_mlp_init.argtypes = [_vp, _u64]
+        L.ptn_mlp_set_params.argtypes = [_vp, _pf]
+        L.ptn_mlp_get_params.argtypes = [_vp, _pf]
+        L.ptn_mlp_forward.argtypes = [_vp, _i, _pf, _pf]
+        L.ptn_mlp_loss_grad.argtypes = [_vp, _i, _pf, _pf, _pu8, _pd, _pf]
+        L.ptn_mlp_data_clear.argtypes = [_vp]
+        L.ptn_mlp_data_add.argtypes = [_vp, _i, _pf, _pf, _pu8]
+        L.ptn_mlp_data_size.argtypes = [_vp, ctypes.POINTER(_i64)]
+        L.ptn_mlp_train.argtypes = [_vp, _i, _i, _d, _u64, _pd]
+        L.ptn_mlp_data_get.argtypes = [_vp, _i64, _i, _pf, _pf, _pu8]
         _lib = L
     return _lib
 
 
 def _p(a):
     return a.ctypes.data_as(_pd)
+
+
+def _pf32(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _pu8(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte))
 
 
 def _dims(dims):
@@ -224,12 +247,69 @@ class Handle:
             log.append({"f_value": r.f_value, "rel_cost": r.rel_cost, "f_accel": r.f_accel,
                         "accel_tried": r.accel_tried, "accel_accepted": r.accel_accepted,
                         "nnls_iters": list(r.nnls_iters[:N]), "zero_col_fallback": list(r.zero_col_fallback[:N]),
-                        "lipschitz": list(r.lipschitz[:N]), "mu": list(r.mu[:N]), "t_end": r.t_end})
+                        "lipschitz": list(r.lipschitz[:N]), "mu": list(r.mu[:N]), "t_end": r.t_end,
+                        "warm_rows": list(r.warm_rows[:N])})
         return unpack(out, dims, R), status, log
 
     def cpd_dimtree(self, R, init, **kw):
         kw.setdefault("mttkrp_form", MTTKRP_DIMTREE)
         return self.cpd(R, init, **kw)
+
+    # -- the NNLS warm-start network --
+    def mlp_init(self, seed):
+        self._check(lib().ptn_mlp_init(self._h, seed))
+
+    def mlp_params(self):
+        w = np.empty(MLP_PARAMS, dtype=np.float32)
+        self._check(lib().ptn_mlp_get_params(self._h, _pf32(w)))
+        return w
+
+    def mlp_set_params(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        assert w.size == MLP_PARAMS
+        self._check(lib().ptn_mlp_set_params(self._h, _pf32(w)))
+
+    def mlp_forward(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.empty((X.shape[0], MLP_OUT), dtype=np.float32)
+        self._check(lib().ptn_mlp_forward(self._h, X.shape[0], _pf32(X), _pf32(Y)))
+        return Y
+
+    def mlp_loss_grad(self, X, T, rank):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        rk = np.ascontiguousarray(rank, dtype=np.uint8)
+        loss = _d()
+        g = np.empty(MLP_PARAMS, dtype=np.float32)
+        self._check(lib().ptn_mlp_loss_grad(self._h, X.shape[0], _pf32(X), _pf32(T), _pu8(rk), ctypes.byref(loss),
+                                            _pf32(g)))
+        return loss.value, g
+
+    def mlp_data_clear(self):
+        self._check(lib().ptn_mlp_data_clear(self._h))
+
+    def mlp_data_add(self, X, T, rank):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        rk = np.ascontiguousarray(rank, dtype=np.uint8)
+        self._check(lib().ptn_mlp_data_add(self._h, X.shape[0], _pf32(X), _pf32(T), _pu8(rk)))
+
+    def mlp_data_size(self):
+        n = _i64()
+        self._check(lib().ptn_mlp_data_size(self._h, ctypes.byref(n)))
+        return n.value
+
+    def mlp_data_get(self, first, n):
+        X = np.empty((n, MLP_IN), dtype=np.float32)
+        T = np.empty((n, MLP_OUT), dtype=np.float32)
+        rk = np.empty(n, dtype=np.uint8)
+        self._check(lib().ptn_mlp_data_get(self._h, first, n, _pf32(X), _pf32(T), _pu8(rk)))
+        return X, T, rk
+
+    def mlp_train(self, epochs, batch=1024, lr=1e-3, seed=0):
+        out = np.empty(epochs, dtype=np.float64)
+        self._check(lib().ptn_mlp_train(self._h, epochs, batch, lr, seed, _p(out)))
+        return out
 
     # -- batched independent order-3 problems --
     def _set_batch_shape(self, dims, ranks):

@@ -45,10 +45,32 @@ def test_options_defaults_match_reference_defaults():
     assert o.max_inner == 50
 
 
-def test_struct_layouts_match_header():
-    # sizes of the C structs as compiled (x86-64 SysV): catches a field added on one side only
-    assert ctypes.sizeof(ptn_hip.Options) == 4 + 4 + 8 * 3 + 4 * 4 + 4 * 8 + 4 * 4
-    assert ctypes.sizeof(ptn_hip.Result) == 8 * 3 + 4 * 3 + 4 + 8
+def test_struct_layouts_match_header(tmp_path):
+    """sizeof and field offsets of every C ABI struct, as the host compiler lays them out
+    from the header, equal the ctypes mirrors (catches a field added on one side only)."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no host C compiler")
+    mirrors = {"ptn_options": ptn_hip.Options, "ptn_result": ptn_hip.Result, "ptn_sweep_log": ptn_hip.SweepLog,
+               "ptn_batch_log": ptn_hip.BatchLog, "ptn_batch_report": ptn_hip.BatchReport}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "ptn_hip.h"', "int main(void) {"]
+    for cname, cls in mirrors.items():
+        lines.append('printf("%s %%zu\\n", sizeof(%s));' % (cname, cname))
+        for f in cls._fields_:
+            lines.append('printf("%s.%s %%zu\\n", offsetof(%s, %s));' % (cname, f[0], cname, f[0]))
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = dict(l.split() for l in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                 text=True).stdout.splitlines())
+    for cname, cls in mirrors.items():
+        assert int(got[cname]) == ctypes.sizeof(cls), cname
+        for f in cls._fields_:
+            assert int(got["%s.%s" % (cname, f[0])]) == getattr(cls, f[0]).offset, (cname, f[0])
 
 
 def test_invalid_handle_is_an_error_not_a_crash():

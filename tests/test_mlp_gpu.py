@@ -1,0 +1,142 @@
+"""The NNLS warm-start network on the GPU: the BF16 forward and backward kernels against
+the FP64 oracle (with the same BF16 operands), the harvested features against the
+oracle's, the fallback invariant (a network whose proposals are rejected leaves the
+decomposition bit-identical to the plain one), and what a trained network changes on a
+held-out problem."""
+import numpy as np
+import pytest
+
+import mlp_oracle as m
+import oracle as o
+import ptn_hip as P
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-test]", *a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def h():
+    say("create handle")
+    with P.Handle(0) as hd:
+        yield hd
+
+
+BLOCKS = [0, m.H * m.IN, m.H * m.IN + m.H, m.H * m.IN + m.H + m.H * m.H, m.H * m.IN + 2 * m.H + m.H * m.H,
+          m.N_PARAMS - m.R_MAX, m.N_PARAMS]
+
+
+def _data(n, seed):
+    rng = np.random.default_rng(seed)
+    X = rng.standard_normal((n, m.IN)) * 0.3
+    T = rng.standard_normal((n, m.R_MAX))
+    rank = rng.integers(1, m.R_MAX + 1, n)
+    return X, T, rank
+
+
+def test_forward_matches_oracle(h):
+    h.mlp_init(5)
+    w = h.mlp_params().astype(np.float64)
+    X, _, _ = _data(100, 0)
+    Y = h.mlp_forward(X)
+    Yo, _ = m.forward(w, X, bf16=True)
+    say("forward rel err", P.rel_err(Y, Yo))
+    assert P.rel_err(Y, Yo) < 2e-3          # FP32 accumulation vs FP64 on the same BF16 operands
+
+
+def test_gradient_matches_oracle(h):
+    h.mlp_init(6)
+    w = h.mlp_params().astype(np.float64)
+    X, T, rank = _data(77, 1)                # not a multiple of 32: padded rows must not count
+    loss, g = h.mlp_loss_grad(X, T, rank)
+    lo, go = m.loss_grad(w, X, T, rank, bf16=True)
+    assert loss == pytest.approx(lo, rel=1e-4)
+    for a, b in zip(BLOCKS, BLOCKS[1:]):
+        err = P.rel_err(g[a:b].astype(np.float64), go[a:b])
+        say("gradient block at", a, "rel err", err)
+        assert err < 5e-3
+
+
+def test_harvested_features_match_oracle(h):
+    dims, R = [12, 10, 8], 3
+    X, _ = o.make_tensor(dims, R, seed=3, snr_db=30.0)
+    init = o.gen_factors(dims, R, seed=4)
+    h.set_tensor(X)
+    h.mlp_data_clear()
+    h.cpd(R, init, max_iter=1, threshold_error=0.0, mlp_harvest=1)
+    assert h.mlp_data_size() == sum(dims)
+    Xd, Td, rk = h.mlp_data_get(0, dims[0])      # the first update: mode 0 of sweep 1
+    G = [o.gram(A) for A in init]
+    V = G[1] * G[2]
+    M = o.mttkrp(X, init, 0)
+    Anew, _, _, _ = o.nesterov(V, M, init[0])
+    Xo, To, _ = m.features(V, M, init[0], Anew)
+    Xo = m.bf16_round(Xo)
+    assert np.all(rk == R)
+    assert np.mean(Xd == Xo) > 0.999 and np.max(np.abs(Xd - Xo)) <= 1e-2 * np.max(np.abs(Xo))
+    assert P.rel_err(Td.astype(np.float64), To) < 1e-6
+
+
+def test_rejected_proposals_leave_the_plain_result_bitwise(h):
+    """A network whose outputs are NaN fails the finiteness check on every row: every NNLS
+    starts from the previous row, exactly as without the network."""
+    dims, R = [60, 50, 40], 6
+    X, _ = o.make_tensor(dims, R, seed=8, snr_db=30.0)
+    init = o.gen_factors(dims, R, seed=9)
+    h.set_tensor(X)
+    w = np.zeros(m.N_PARAMS, dtype=np.float32)
+    w[-m.R_MAX:] = np.nan
+    h.mlp_set_params(w)
+    for form in (P.MTTKRP_STANDARD, P.MTTKRP_DIMTREE):
+        Fw, sw, lw = h.cpd(R, init, max_iter=12, threshold_error=0.0, warm_start=1, mttkrp_form=form)
+        Fp, sp, lp = h.cpd(R, init, max_iter=12, threshold_error=0.0, mttkrp_form=form)
+        assert all(sum(l["warm_rows"]) == 0 for l in lw)
+        assert [l["nnls_iters"] for l in lw] == [l["nnls_iters"] for l in lp]
+        assert sw["f_value"] == sp["f_value"]
+        for a, b in zip(Fw, Fp):
+            assert np.array_equal(a, b)
+
+
+def test_trained_network_on_a_held_out_problem(h):
+    """Harvest (inputs, Nesterov solution) pairs from two 100^3 R = 10 runs, train, and
+    decompose a third tensor with and without the warm start.  Reported: NNLS steps in
+    total, rows started from the network, and the sweep at which each run first reaches
+    the plain run's final error.  Asserted: training lowers the loss, the network is
+    used, and the final error stays that of the plain solver (the start changes, the
+    subproblem and its stopping test do not)."""
+    dims, R = [100, 100, 100], 10
+    h.mlp_data_clear()
+    for seed in (0, 10):
+        h.generate_tensor(dims, R, seed=seed, snr_db=30.0)
+        h.cpd(R, h.generate_factors(dims, R, seed=seed + 1), max_iter=30, threshold_error=0.0, mlp_harvest=1)
+    h.mlp_init(7)
+    loss = h.mlp_train(20, batch=256, lr=1e-3, seed=1)
+    say("training loss per epoch: first %.3e last %.3e, %d rows" % (loss[0], loss[-1], h.mlp_data_size()))
+    assert loss[-1] < 0.2 * loss[0]
+    h.generate_tensor(dims, R, seed=20, snr_db=30.0)
+    init = h.generate_factors(dims, R, seed=21)
+    Fp, sp, lp = h.cpd(R, init, max_iter=40, threshold_error=0.0)
+    Fw, sw, lw = h.cpd(R, init, max_iter=40, threshold_error=0.0, warm_start=1)
+    ip = sum(sum(l["nnls_iters"]) for l in lp)
+    iw = sum(sum(l["nnls_iters"]) for l in lw)
+    rows = sum(sum(l["warm_rows"]) for l in lw)
+    tgt = sp["rel_cost"] + 1e-9
+    kp = next(i + 1 for i, l in enumerate(lp) if l["rel_cost"] <= tgt)
+    kw = next((i + 1 for i, l in enumerate(lw) if l["rel_cost"] <= tgt), None)
+    say("NNLS steps: plain %d, warm %d (saved %d); warm rows %d of %d; sweeps to the plain final error: "
+        "plain %d, warm %s; final rel error plain %.12f warm %.12f"
+        % (ip, iw, ip - iw, rows, 40 * sum(dims), kp, kw, sp["rel_cost"], sw["rel_cost"]))
+    assert rows > 0
+    assert abs(sw["rel_cost"] - sp["rel_cost"]) <= 1e-6 * sp["rel_cost"]
+
+
+def test_warm_start_needs_a_network():
+    with P.Handle(0) as h2:
+        h2.set_tensor(np.ones((4, 5, 6), order="F"))
+        with pytest.raises(P.PtnError):
+            h2.cpd(2, [np.ones((I, 2)) for I in (4, 5, 6)], warm_start=1)
+        h2.mlp_init(1)
+        with pytest.raises(P.PtnError):
+            h2.mlp_train(1, batch=32)             # empty training set
