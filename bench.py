@@ -238,6 +238,85 @@ def bench_batch(args):
     print(json.dumps(out))
 
 
+def _time_to(log, target):
+    """Seconds from the start of the run to the end of the first sweep whose relative
+    cost is <= target (None if never)."""
+    for rec in log:
+        if rec["rel_cost"] <= target:
+            return rec["t_end"]
+    return None
+
+
+def bench_warm(args):
+    """The learned NNLS warm start.  The network is trained on (subproblem, Nesterov
+    solution) pairs harvested on the device from runs of the 100^3 R=10 (50 sweeps) and
+    800^3 R=32 (30 sweeps) configurations, then used on the 1000^3 FP32 tensor (true rank
+    64, U[0,1) factors seed 4, SNR 15 dB) at R = 64.  Reported: NNLS steps with and without
+    the warm start in the chosen form, rows that took the network's start, and the time
+    each run needs to reach the plain FP64 GPU path's final relative error (+1e-6)."""
+    h = P.Handle(0)
+    sweeps = args.warm_sweeps
+    t = time.time()
+    h.mlp_data_clear()
+    h.generate_tensor([100] * 3, 10, seed=0, snr_db=30.0)
+    h.cpd(10, h.generate_factors([100] * 3, 10, seed=1), max_iter=50, threshold_error=0.0, mlp_harvest=1)
+    h.generate_tensor(DIMS, R, seed=0, snr_db=20.0, storage=P.TENSOR_BOTH)
+    h.cpd(R, h.generate_factors(DIMS, R, seed=1), max_iter=30, threshold_error=0.0, mttkrp_form=P.MTTKRP_DIMTREE,
+          arith=P.ARITH_BF16X3, fuse_accel=1, mlp_harvest=1)
+    rows = h.mlp_data_size()
+    log("harvested %d pairs in %.1f s" % (rows, time.time() - t))
+    t = time.time()
+    h.mlp_init(4)
+    losses = h.mlp_train(args.warm_epochs, batch=1024, lr=1e-3, seed=4)
+    train_s = time.time() - t
+    log("trained %d epochs in %.2f s: loss %.3e -> %.3e" % (args.warm_epochs, train_s, losses[0], losses[-1]))
+    dims5, R5 = [1000] * 3, 64
+    t = time.time()
+    h.generate_tensor(dims5, R5, seed=4, snr_db=15.0, storage=P.TENSOR_BOTH)
+    init = h.generate_factors(dims5, R5, seed=5)
+    frob = h.norm_sq()
+    log("1000^3 tensor generated in %.1f s" % (time.time() - t))
+    common = dict(max_iter=sweeps, threshold_error=0.0)
+    Fr, sr, lr_ = h.cpd(R5, init, mttkrp_form=P.MTTKRP_STANDARD, arith=P.ARITH_F64, **common)
+    e_ref = h.cost_explicit(Fr) / math.sqrt(frob)
+    chosen = dict(mttkrp_form=P.MTTKRP_DIMTREE, arith=P.ARITH_BF16X3, fuse_accel=1, **common)
+    Fp, sp, lp = h.cpd(R5, init, **chosen)
+    Fw, sw, lw = h.cpd(R5, init, warm_start=1, **chosen)                       # gate: KKT residual <= 1e-6
+    Fl, sl, ll = h.cpd(R5, init, warm_start=1, warm_kkt_tol=5e-2, **chosen)    # loose gate, for comparison
+    h.close()
+    target = e_ref + 1e-6
+    out = {}
+    for name, F, st, lg in (("plain_start", Fp, sp, lp), ("warm_start", Fw, sw, lw),
+                            ("warm_start_loose_gate_5e-2", Fl, sl, ll)):
+        out[name] = {"nnls_steps": int(sum(sum(r["nnls_iters"]) for r in lg)),
+                     "warm_rows": int(sum(sum(r["warm_rows"]) for r in lg)),
+                     "seconds": lg[-1]["t_end"], "sweeps_per_s": len(lg) / lg[-1]["t_end"],
+                     "final_rel_cost": st["rel_cost"],
+                     "seconds_to_fp64_final_error": _time_to(lg, target)}
+    ref_t = lr_[-1]["t_end"]
+    saved = out["plain_start"]["nnls_steps"] - out["warm_start"]["nnls_steps"]
+    tp, tw = out["plain_start"]["seconds_to_fp64_final_error"], out["warm_start"]["seconds_to_fp64_final_error"]
+    log("fp64 ref %.9f in %.2f s; plain start %s s, warm start %s s; NNLS steps saved %d"
+        % (e_ref, ref_t, tp, tw, saved))
+    res = {
+        "metric": "learned NNLS warm start (1000^3, R=64): NNLS steps saved and time to the FP64 path's final error",
+        "value": (tp / tw) if (tp and tw) else None, "unit": "x (time to target, plain start / warm start)",
+        "n_gpus": 1, "steps": sweeps, "warmup": 0, "ms_per_step": 1e3 * out["warm_start"]["seconds"] / sweeps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (device Philox4x32-10): training pairs from the 100^3 R=10 SNR 30 dB and 800^3 R=32 "
+                "SNR 20 dB runs; evaluation tensor 1000^3 true rank 64 U[0,1) seed 4, SNR 15 dB, init seed 5",
+        "config": {"workload": "dense 3-way 1000x1000x1000 FP32 (+FP64 copy), R=64, %d sweeps, chosen form "
+                               "(dimension tree, bf16x3, fused line search) with and without the warm start"
+                               % sweeps,
+                   "training_pairs": rows, "epochs": args.warm_epochs, "train_seconds": train_s,
+                   "train_loss_first_last": [float(losses[0]), float(losses[-1])],
+                   "fp64_plain_final_rel_error": e_ref, "fp64_plain_seconds": ref_t, "target": target,
+                   "nnls_steps_saved": saved, **out},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -253,6 +332,10 @@ def main():
     ap.add_argument("--batch-sweeps", type=int, default=BATCH_SWEEPS)
     ap.add_argument("--batch-block", type=int, default=0, help="sweep kernel block size (0: autotune)")
     ap.add_argument("--batch-grid", type=int, default=0, help="sweep kernel grid (0: autotune)")
+    ap.add_argument("--warm", action="store_true",
+                    help="only run the learned warm-start experiment (train, then 1000^3 R=64 with/without) and exit")
+    ap.add_argument("--warm-sweeps", type=int, default=30)
+    ap.add_argument("--warm-epochs", type=int, default=8)
     ap.add_argument("--gemm-node", action="store_true",
                     help="only time the order-4 GEMM node (200^4, R=64, FP32 MFMA vs bf16x3) and exit")
     args = ap.parse_args()
@@ -261,6 +344,9 @@ def main():
         return
     if args.batch:
         bench_batch(args)
+        return
+    if args.warm:
+        bench_warm(args)
         return
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)

@@ -63,8 +63,8 @@ typedef struct {
   int      fuse_accel;          /* dimension tree + FP32: fold the acceleration pass into the next left-node pass */
   /* the learned NNLS warm start (no reference counterpart) */
   int      warm_start;          /* 1: start each NNLS from the handle's network where it passes the KKT test */
-  double   warm_kkt_tol;        /* a row takes the network's start if its relative KKT residual is <= this
-                                   or below the previous row's (default 1e-6) */
+  double   warm_kkt_tol;        /* a row takes the network's start if its relative KKT residual
+                                   max|min(L a, a Zt - Wt)| / max|Wt| is <= this (default 1e-6) */
   int      mlp_harvest;         /* 1: append every NNLS (inputs, solution) pair to the handle's training set */
 } ptn_options;
 

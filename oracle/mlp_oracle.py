@@ -109,13 +109,15 @@ def loss_grad(w, X, T, rank, bf16=False):
     return loss, g
 
 
-def kkt_residual(V, lam, M, Aprev, A):
-    """Relative natural KKT residual of start rows A for the subproblem NesterovMNLS solves
-    (Zt = V + lam I, Wt = M + lam Aprev): max_c |min(a_c, (A Zt - Wt)_c)| / max_c |Wt_c| per row."""
+def kkt_residual(V, lam, L, M, Aprev, A):
+    """Relative KKT residual of start rows A for the subproblem NesterovMNLS solves
+    (Zt = V + lam I, Wt = M + lam Aprev, Lipschitz constant L): the projected-gradient
+    fixed-point residual min(a, g / L), g = a Zt - Wt, relative to max|Wt| / L, i.e.
+    max_c |min(L a_c, g_c)| / max_c |Wt_c| per row."""
     R = V.shape[0]
     Zt = V + lam * np.eye(R)
     Wt = M + lam * Aprev
     G = A @ Zt - Wt
     den = np.max(np.abs(Wt), axis=1)
     den = np.where(den > 0, den, 1.0)
-    return np.max(np.abs(np.minimum(A, G)), axis=1) / den
+    return np.max(np.abs(np.minimum(L * A, G)), axis=1) / den

@@ -116,53 +116,51 @@ __global__ __launch_bounds__(256) void k_mlp_features(int64_t I, int R, int ldr,
 }
 
 // ---------------------------------------------------------------------------
-// The warm-start decision (one wave per row, lane = column).  With Zt = V + lambda I and
-// wt = m + lambda a_prev (the subproblem NesterovMNLS solves, anchor unchanged), the
-// relative KKT residual of a start a is r(a) = max_c |min(a_c, (a Zt - wt)_c)| / max_c |wt_c|.
+// The warm-start decision (one wave per row, lane = column).  With Zt = V + lambda I,
+// wt = m + lambda a_prev (the subproblem NesterovMNLS solves, anchor unchanged) and L its
+// Lipschitz constant, the relative KKT residual of a start a is the projected-gradient
+// fixed-point residual min(a, g / L), g = a Zt - wt, relative to the solution's scale
+// max|wt| / L:  r(a) = max_c |min(L a_c, g_c)| / max_c |wt_c|.
 // The row starts from the network's proposal a_p = max(y, 0) * scale when y is finite and
-// r(a_p) <= kkt_tol or r(a_p) < r(a_prev); otherwise from a_prev (the fallback, which is
-// exactly the reference's start).  S receives the start rows; count the rows that took a_p.
+// r(a_p) <= kkt_tol; otherwise from a_prev (the fallback, which is exactly the reference's
+// start).  S receives the start rows; count the rows that took a_p.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_warm_select(int64_t I, int R, int ldr, const double *__restrict__ V,
                                                      const NnlsParams *__restrict__ prm, const double *__restrict__ M,
                                                      const double *__restrict__ A, const float *__restrict__ Y,
                                                      const float *__restrict__ scale, double kkt_tol,
                                                      double *__restrict__ S, int *count) {
-  __shared__ double buf[4][2][kMlpR];
+  __shared__ double buf[4][kMlpR];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t row = (int64_t)blockIdx.x * 4 + wave;
   if (row >= I) return;
   const bool on = lane < R;
-  const double lambda = prm->lambda;
+  const double lambda = prm->lambda, L = prm->L;
   const double ap = on ? A[row * ldr + lane] : 0.0;
   const float y = on ? Y[row * kMlpR + lane] : 0.f;
   const bool fin = !on || isfinite(y);
   const double aw = (on && y > 0.f) ? (double)y * (double)scale[row] : 0.0;
-  buf[wave][0][lane] = ap;
-  buf[wave][1][lane] = aw;
+  buf[wave][lane] = aw;
   __builtin_amdgcn_wave_barrier();
-  double gp = 0.0, gw = 0.0;
+  double gw = 0.0;
   if (on) {
     for (int k = 0; k < R; ++k) {
       const double z = V[k * R + lane] + (k == lane ? lambda : 0.0);
-      gp = fma(buf[wave][0][k], z, gp);
-      gw = fma(buf[wave][1][k], z, gw);
+      gw = fma(buf[wave][k], z, gw);
     }
   }
   const double wt = on ? M[row * ldr + lane] + lambda * ap : 0.0;
-  gp -= wt;
   gw -= wt;
-  double rp = fabs(fmin(ap, gp)), rw = fabs(fmin(aw, gw)), sw = fabs(wt);
-  if (!on) rp = rw = sw = 0.0;
+  double rw = fabs(fmin(L * aw, gw)), sw = fabs(wt);
+  if (!on) rw = sw = 0.0;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
-    rp = fmax(rp, __shfl_xor(rp, o, 64));
     rw = fmax(rw, __shfl_xor(rw, o, 64));
     sw = fmax(sw, __shfl_xor(sw, o, 64));
   }
   const bool all_fin = __ballot(!fin) == 0ull;
   const double den = sw > 0.0 ? sw : 1.0;
-  const bool take = all_fin && (rw / den <= kkt_tol || rw < rp);
+  const bool take = all_fin && rw / den <= kkt_tol;
   if (lane < ldr) S[row * ldr + lane] = take ? (on ? aw : 0.0) : (lane < R ? ap : A[row * ldr + lane]);
   if (take && lane == 0) atomicAdd(count, 1);
 }

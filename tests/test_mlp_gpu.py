@@ -103,9 +103,11 @@ def test_trained_network_on_a_held_out_problem(h):
     """Harvest (inputs, Nesterov solution) pairs from two 100^3 R = 10 runs, train, and
     decompose a third tensor with and without the warm start.  Reported: NNLS steps in
     total, rows started from the network, and the sweep at which each run first reaches
-    the plain run's final error.  Asserted: training lowers the loss, the network is
-    used, and the final error stays that of the plain solver (the start changes, the
-    subproblem and its stopping test do not)."""
+    the plain run's final error.  Asserted: training lowers the loss; under the default
+    gate (relative KKT residual <= 1e-6) a row takes the network's start only when the
+    proposal is that close to the subproblem's solution, so the run ends at the plain
+    solver's error (bit-identical when no row qualifies); under a loose gate the
+    proposals are taken and the run stays finite."""
     dims, R = [100, 100, 100], 10
     h.mlp_data_clear()
     for seed in (0, 10):
@@ -118,18 +120,25 @@ def test_trained_network_on_a_held_out_problem(h):
     h.generate_tensor(dims, R, seed=20, snr_db=30.0)
     init = h.generate_factors(dims, R, seed=21)
     Fp, sp, lp = h.cpd(R, init, max_iter=40, threshold_error=0.0)
-    Fw, sw, lw = h.cpd(R, init, max_iter=40, threshold_error=0.0, warm_start=1)
     ip = sum(sum(l["nnls_iters"]) for l in lp)
-    iw = sum(sum(l["nnls_iters"]) for l in lw)
-    rows = sum(sum(l["warm_rows"]) for l in lw)
     tgt = sp["rel_cost"] + 1e-9
     kp = next(i + 1 for i, l in enumerate(lp) if l["rel_cost"] <= tgt)
-    kw = next((i + 1 for i, l in enumerate(lw) if l["rel_cost"] <= tgt), None)
-    say("NNLS steps: plain %d, warm %d (saved %d); warm rows %d of %d; sweeps to the plain final error: "
-        "plain %d, warm %s; final rel error plain %.12f warm %.12f"
-        % (ip, iw, ip - iw, rows, 40 * sum(dims), kp, kw, sp["rel_cost"], sw["rel_cost"]))
-    assert rows > 0
-    assert abs(sw["rel_cost"] - sp["rel_cost"]) <= 1e-6 * sp["rel_cost"]
+    for tol in (1e-6, 5e-2):
+        Fw, sw, lw = h.cpd(R, init, max_iter=40, threshold_error=0.0, warm_start=1, warm_kkt_tol=tol)
+        iw = sum(sum(l["nnls_iters"]) for l in lw)
+        rows = sum(sum(l["warm_rows"]) for l in lw)
+        kw = next((i + 1 for i, l in enumerate(lw) if l["rel_cost"] <= tgt), None)
+        say("gate %g: NNLS steps plain %d, warm %d (saved %d); warm rows %d of %d; sweeps to the plain final "
+            "error: plain %d, warm %s; final rel error plain %.12f warm %.12f"
+            % (tol, ip, iw, ip - iw, rows, 40 * sum(dims), kp, kw, sp["rel_cost"], sw["rel_cost"]))
+        if tol == 1e-6:
+            assert abs(sw["rel_cost"] - sp["rel_cost"]) <= 1e-6 * sp["rel_cost"]
+            if rows == 0:
+                assert [l["nnls_iters"] for l in lw] == [l["nnls_iters"] for l in lp]
+                for a, b in zip(Fw, Fp):
+                    assert np.array_equal(a, b)
+        else:
+            assert rows > 0 and np.isfinite(sw["rel_cost"])
 
 
 def test_warm_start_needs_a_network():
