@@ -17,8 +17,6 @@ namespace {
 
 constexpr int kBR = 16;       // rank padding of a batched problem (rows of kBR doubles)
 constexpr int kBMax = 128;    // largest mode size
-constexpr int kPartial = 2 * kBMax * kBR;   // doubles available for the mode-0 group partials
-constexpr int kLoads = 8;                   // tensor loads a lane issues before using them
 
 struct BatchLds {
   double F[2][kBMax * kBR];   // the two factors a contraction multiplies by; during a solve
@@ -35,6 +33,7 @@ struct BatchLds {
   double fval;
   int colnz[kBR];
   int iters[3];
+  int seg[2 * BatchArgs::kMaxThreads / 64];   // partial MTTKRP tiles per wave
 };
 
 // threadIdx.x behind an empty asm statement: index arithmetic derived from it is redone
@@ -93,132 +92,143 @@ __device__ __forceinline__ void gram_block(const double *A, int I, int R, double
   __syncthreads();
 }
 
-// Mode-0 MTTKRP, first stage.  Thread (i0, g) of the first G * I0 threads takes the
-// slabs i2 = g, g + G, ...: T = sum_i1 X(i0, i1, i2) F1[i1] (the loads of one i1 are
-// contiguous over i0), acc += T * F2[i2].
-template <int RP>
-__device__ __forceinline__ void mttkrp0_partial(const float *X, int I0, int I1, int I2, int G, const double *F1,
-                                                const double *F2, double (&acc)[RP]) {
-  const int t = tid(), g = t / I0, i0 = t - g * I0;
-#pragma unroll
-  for (int r = 0; r < RP; ++r) acc[r] = 0.0;
-  if (g >= G) return;
-  const int s2 = I0 * I1;
-  for (int i2 = g; i2 < I2; i2 += G) {
-    const float *x = X + (i0 + s2 * i2);
-    double T[RP];
-#pragma unroll
-    for (int r = 0; r < RP; ++r) T[r] = 0.0;
-    int i1 = 0;
-    for (; i1 + kLoads <= I1; i1 += kLoads) {     // kLoads tensor loads in flight
-      float xs[kLoads];
-#pragma unroll
-      for (int u = 0; u < kLoads; ++u) xs[u] = x[I0 * (i1 + u)];
-#pragma unroll
-      for (int u = 0; u < kLoads; ++u) {
-        const double xv = (double)xs[u];
-        const double *f = F1 + (i1 + u) * kBR;
-#pragma unroll
-        for (int r = 0; r < RP; ++r) T[r] = fma(xv, f[r], T[r]);
-      }
-    }
-    for (; i1 < I1; ++i1) {
-      const double xv = (double)x[I0 * i1];
-      const double *f = F1 + i1 * kBR;
-#pragma unroll
-      for (int r = 0; r < RP; ++r) T[r] = fma(xv, f[r], T[r]);
-    }
-    const double *f2 = F2 + i2 * kBR;
-#pragma unroll
-    for (int r = 0; r < RP; ++r) acc[r] = fma(T[r], f2[r], acc[r]);
-  }
-}
+// The MTTKRP of one mode as 16 x 16 output tiles on v_mfma_f64_16x16x4f64:
+//   M[row][r] = sum_k X(row, k) KR[k][r],  KR[k][r] = Fa[ia][r] Fb[ib][r],  k = ia + Ia ib,
+//   MODE 0: row = i0, X offset row + I0 k,                 Fa = A1, Fb = A2
+//   MODE 1: row = i1, X offset I0 row + ia + I0 I1 ib,      Fa = A0, Fb = A2
+//   MODE 2: row = i2, X offset I0 I1 row + k,               Fa = A0, Fb = A1
+// One MFMA per step of 4 k.  Lane l holds A = X(row l % 16, k = 4 s + l / 16) (FP32 widened
+// to FP64, 0 past the tensor), B = KR[4 s + l / 16][l % 16], and D rows l / 16 + 4 j, column
+// l % 16.  The T tiles x KS steps are split evenly over the nw waves (stream-K): a wave
+// stores the tiles its range covers whole straight into M and keeps the (at most two)
+// partial ones in registers for mttkrp_mfma_partials / mttkrp_mfma_reduce, which sum them
+// in wave order (a fixed order: the result does not depend on the grid).
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+constexpr int kSteps = 8;                   // k-steps whose tensor loads a lane issues together
 
-// Mode-0 MTTKRP, second stage: the G group partials go through LDS (P, laid out
-// [r][g][i0]) and are summed in group order into M; columns RP.. of M are zeroed.
-template <int BS, int RP>
-__device__ __forceinline__ void mttkrp0_combine(int I0, int G, const double (&acc)[RP], double *P, double *M) {
-  const int t = tid(), g = t / I0, i0 = t - g * I0;
-  if (g < G) {
-#pragma unroll
-    for (int r = 0; r < RP; ++r) P[(r * G + g) * I0 + i0] = acc[r];
-  }
-  __syncthreads();
-  for (int e = t; e < I0 * kBR; e += BS) {
-    const int i = e / kBR, r = e - i * kBR;
-    double s = 0.0;
-    if (r < RP) {
-      s = P[r * G * I0 + i];
-      for (int q = 1; q < G; ++q) s = s + P[(r * G + q) * I0 + i];
-    }
-    M[e] = s;
-  }
-  __syncthreads();
-}
+struct MfmaPart {
+  f64x4 acc0, acc1;
+  int tile0, tile1;
+};
 
-// Modes 1 and 2: wave w < nw takes the output rows io = w, w + nw, ...  Its lanes split
-// the row's slab into i0 (the contiguous index; a lane takes i0 and i0 + 64 when
-// I0 > 64) and phases of the inner index j (when I0 < 64):
-// T = sum_j X F_j[j], P = sum_i0 T * F0[i0], P summed over the wave.
-// MODE 1: out i1, j = i2 (element i0 + I0 (io + I1 j)); MODE 2: out i2, j = i1.
-template <int RP, int MODE>
-__device__ __forceinline__ void mttkrp_rows(const float *X, int I0, int I1, int I2, const double *F0, const double *Fj, double *M,
-                            int nw) {
+template <int MODE>
+__device__ __forceinline__ void mttkrp_mfma(const float *X, int I0, int I1, int I2, const double *Fa,
+                                            const double *Fb, double *M, int nw, MfmaPart &part) {
   const int t = tid(), lane = t & 63, wave = t >> 6;
+  part.tile0 = part.tile1 = -1;
   if (wave >= nw) return;
-  const int Iout = MODE == 1 ? I1 : I2, Iin = MODE == 1 ? I2 : I1;
-  const int so = MODE == 1 ? I0 : I0 * I1;
-  const int sj = MODE == 1 ? I0 * I1 : I0;
-  int JP = 1, i0 = lane, ph = 0;
-  if (I0 < 64) {
-    JP = 64 / I0;
-    ph = lane / I0;
-    i0 = lane - ph * I0;
-  }
-  const bool live = ph < JP;
-  for (int io = wave; io < Iout; io += nw) {
-    double P[RP];
+  const int Iout = MODE == 0 ? I0 : MODE == 1 ? I1 : I2;
+  const int Ia = MODE == 0 ? I1 : I0, Ib = MODE == 2 ? I1 : I2;
+  const int K = Ia * Ib, KS = (K + 3) / 4, T = (Iout + 15) / 16;
+  const int rs = MODE == 0 ? 1 : MODE == 1 ? I0 : I0 * I1;
+  const int64_t S = (int64_t)T * KS;
+  const int64_t s0 = (int64_t)wave * S / nw, s1 = (int64_t)(wave + 1) * S / nw;
+  const int m = lane & 15, kk = lane >> 4;
+  int slot = 0;
+  for (int64_t s = s0; s < s1;) {
+    const int tile = (int)(s / KS);
+    const int64_t tb = (int64_t)tile * KS, te = tb + KS, e = te < s1 ? te : s1;
+    const int row = tile * 16 + m;
+    const bool rowok = row < Iout;
+    const float *xr = X + (rowok ? row : 0) * rs;
+    int k = 4 * (int)(s - tb) + kk, ia = k % Ia, ib = k / Ia;
+    f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+    int n = (int)(e - s);
+    for (; n > 0;) {
+      const int u_n = n >= kSteps ? kSteps : 1;     // a full batch of steps, or one
+      float xs[kSteps];
+      double bs[kSteps];
 #pragma unroll
-    for (int r = 0; r < RP; ++r) P[r] = 0.0;
-    if (live) {
-      for (int ia = i0; ia < I0; ia += 64) {
-        double T[RP];
-#pragma unroll
-        for (int r = 0; r < RP; ++r) T[r] = 0.0;
-        const float *x = X + (ia + so * io);
-        int j = ph;
-        for (; j + (kLoads - 1) * JP < Iin; j += kLoads * JP) {
-          float xs[kLoads];
-#pragma unroll
-          for (int u = 0; u < kLoads; ++u) xs[u] = x[sj * (j + u * JP)];
-#pragma unroll
-          for (int u = 0; u < kLoads; ++u) {
-            const double xv = (double)xs[u];
-            const double *f = Fj + (j + u * JP) * kBR;
-#pragma unroll
-            for (int r = 0; r < RP; ++r) T[r] = fma(xv, f[r], T[r]);
+      for (int u = 0; u < kSteps; ++u) {
+        if (u < u_n) {
+          const bool v = k < K;
+          const int off = MODE == 0 ? I0 * k : MODE == 1 ? ia + I0 * I1 * ib : k;
+          xs[u] = (v && rowok) ? xr[off] : 0.f;
+          bs[u] = v ? Fa[ia * kBR + m] * Fb[ib * kBR + m] : 0.0;
+          k += 4;
+          ia += 4;
+          while (ia >= Ia) {
+            ia -= Ia;
+            ++ib;
           }
         }
-        for (; j < Iin; j += JP) {
-          const double xv = (double)x[sj * j];
-          const double *f = Fj + j * kBR;
-#pragma unroll
-          for (int r = 0; r < RP; ++r) T[r] = fma(xv, f[r], T[r]);
-        }
-        const double *f0 = F0 + ia * kBR;
-#pragma unroll
-        for (int r = 0; r < RP; ++r) P[r] = fma(T[r], f0[r], P[r]);
       }
-    }
 #pragma unroll
-    for (int r = 0; r < RP; ++r) P[r] = wave_sum(P[r]);
-    if (lane < kBR) {
-      double v = 0.0;
-#pragma unroll
-      for (int r = 0; r < RP; ++r) v = lane == r ? P[r] : v;
-      M[io * kBR + lane] = v;
+      for (int u = 0; u < kSteps; ++u)
+        if (u < u_n) acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)xs[u], bs[u], acc, 0, 0, 0);
+      n -= u_n;
     }
+    if (s == tb && e == te) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int r = tile * 16 + kk + 4 * j;
+        if (r < Iout) M[r * kBR + m] = acc[j];
+      }
+    } else {
+      if (slot == 0) {
+        part.acc0 = acc;
+        part.tile0 = tile;
+      } else {
+        part.acc1 = acc;
+        part.tile1 = tile;
+      }
+      slot = 1;
+    }
+    s = e;
   }
+}
+
+// the partial tiles of every wave into P (slots [wave][2][16 x 16]) and their tile ids into
+// seg; P may alias the MTTKRP operands, so this follows a barrier
+__device__ __forceinline__ void mttkrp_mfma_partials(const MfmaPart &part, int nw, double *P, int *seg) {
+  const int t = tid(), lane = t & 63, wave = t >> 6;
+  if (wave >= nw) return;
+  const int m = lane & 15, kk = lane >> 4;
+  if (lane == 0) {
+    seg[2 * wave] = part.tile0;
+    seg[2 * wave + 1] = part.tile1;
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    if (part.tile0 >= 0) P[((2 * wave) * 16 + kk + 4 * j) * kBR + m] = part.acc0[j];
+    if (part.tile1 >= 0) P[((2 * wave + 1) * 16 + kk + 4 * j) * kBR + m] = part.acc1[j];
+  }
+}
+
+// rows of the split tiles: the partials summed in wave order
+template <int BS>
+__device__ __forceinline__ void mttkrp_mfma_reduce(int Iout, int nw, const double *P, const int *seg, double *M) {
+  for (int e = tid(); e < Iout * kBR; e += BS) {
+    const int row = e / kBR, col = e - row * kBR, tile = row >> 4, rin = row & 15;
+    double s = 0.0;
+    bool any = false;
+    for (int w = 0; w < nw; ++w)
+      for (int sl = 0; sl < 2; ++sl)
+        if (seg[2 * w + sl] == tile) {
+          const double v = P[((2 * w + sl) * 16 + rin) * kBR + col];
+          s = any ? s + v : v;
+          any = true;
+        }
+    if (any) M[e] = s;
+  }
+}
+
+// one mode's MTTKRP into S.M with operands S.F[0], S.F[1]: waves [0, nw) compute, `other`
+// runs on the remaining waves between the first two barriers
+template <int BS, int MODE, typename Other>
+__device__ __forceinline__ void mttkrp_block(const float *X, int I0, int I1, int I2, BatchLds &S, int nw,
+                                             Other &&other) {
+  MfmaPart part;
+  if ((int)(tid() >> 6) < nw) mttkrp_mfma<MODE>(X, I0, I1, I2, S.F[0], S.F[1], S.M, nw, part);
+  else {
+    part.tile0 = part.tile1 = -1;
+    other();
+  }
+  __syncthreads();
+  mttkrp_mfma_partials(part, nw, &S.F[0][0], S.seg);
+  __syncthreads();
+  mttkrp_mfma_reduce<BS>(MODE == 0 ? I0 : MODE == 1 ? I1 : I2, nw, &S.F[0][0], S.seg, S.M);
+  __syncthreads();
 }
 
 // NesterovMNLS (NesterovMNLS.hpp:181-234) for the I x R entries of one factor.  Thread
@@ -322,7 +332,7 @@ __device__ __forceinline__ int nnls_block(int I, int R, BatchLds &S, double *Ag,
 // one sweep of one problem
 template <int BS, int RP>
 __device__ __forceinline__ void sweep_problem(const BatchArgs &a, int p, int sweep, BatchLds &S) {
-  constexpr int NW = BS / 64, NT = BS - 64;    // the last wave solves the eigenproblem meanwhile
+  constexpr int NW = BS / 64;                  // the last wave solves the eigenproblem meanwhile
   const BatchDesc &d = a.desc[p];
   BatchProb &st = a.st[p];
   if (st.done) return;                                    // stopped on the threshold earlier
@@ -343,24 +353,17 @@ __device__ __forceinline__ void sweep_problem(const BatchArgs &a, int p, int swe
     load_rows<BS>(n == 0 ? A1 : A0, Ia, S.F[0]);
     load_rows<BS>(n == 2 ? A1 : A2, Ib, S.F[1]);
     __syncthreads();
-    const int G0 = (NT < kPartial / RP ? NT : kPartial / RP) / I0;
-    double acc[RP];
-    if (t >= NT) {
+    auto eig = [&] {
       const int l = t & 63;
       for (int x = l; x < kBR * kBR; x += 64) S.eig.t[x] = S.V[x];
       PTN_WAVE_SYNC();
       double amax, amin;
       eig_extremes_wave<kBR>(R, S.eig, amax, amin);
       if (l == 0) glambda_betas(amax, amin, &S.prm, a.max_inner, S.betas);
-    } else if (n == 0) {
-      mttkrp0_partial<RP>(X, I0, I1, I2, G0, S.F[0], S.F[1], acc);
-    } else if (n == 1) {
-      mttkrp_rows<RP, 1>(X, I0, I1, I2, S.F[0], S.F[1], S.M, NW - 1);
-    } else {
-      mttkrp_rows<RP, 2>(X, I0, I1, I2, S.F[0], S.F[1], S.M, NW - 1);
-    }
-    __syncthreads();
-    if (n == 0) mttkrp0_combine<BS, RP>(I0, G0, acc, &S.F[0][0], S.M);
+    };
+    if (n == 0) mttkrp_block<BS, 0>(X, I0, I1, I2, S, NW - 1, eig);
+    else if (n == 1) mttkrp_block<BS, 1>(X, I0, I1, I2, S, NW - 1, eig);
+    else mttkrp_block<BS, 2>(X, I0, I1, I2, S, NW - 1, eig);
     const int In = n == 0 ? I0 : n == 1 ? I1 : I2;
     const int it = nnls_block<BS, RP>(In, R, S, n == 0 ? A0 : n == 1 ? A1 : A2, a.d1, a.d2, a.max_inner);
     if (t == 0) S.iters[n] = it;
@@ -427,8 +430,7 @@ __device__ __forceinline__ void sweep_problem(const BatchArgs &a, int p, int swe
       __syncthreads();
       gram_block<BS>(S.F[0], I0, R, S.Gh[0]);
       gram_block<BS>(S.F[1], I1, R, S.Gh[1]);
-      mttkrp_rows<RP, 2>(X, I0, I1, I2, S.F[0], S.F[1], S.M, NW);
-      __syncthreads();
+      mttkrp_block<BS, 2>(X, I0, I1, I2, S, NW, [] {});
       double ma = 0.0, vg = 0.0;
       for (int e = t; e < I2 * kBR; e += BS) ma = fma(S.M[e], Ah2[e], ma);
       if (t < kBR * kBR) vg = (S.Gh[1][t] * S.Gh[0][t]) * S.Gh[2][t];
