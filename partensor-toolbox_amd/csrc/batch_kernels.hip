@@ -92,120 +92,149 @@ __device__ __forceinline__ void gram_block(const double *A, int I, int R, double
   __syncthreads();
 }
 
-// The MTTKRP of one mode as 16 x 16 output tiles on v_mfma_f64_16x16x4f64:
+// The MTTKRP of one mode on v_mfma_f64_16x16x4f64:
 //   M[row][r] = sum_k X(row, k) KR[k][r],  KR[k][r] = Fa[ia][r] Fb[ib][r],  k = ia + Ia ib,
 //   MODE 0: row = i0, X offset row + I0 k,                 Fa = A1, Fb = A2
 //   MODE 1: row = i1, X offset I0 row + ia + I0 I1 ib,      Fa = A0, Fb = A2
 //   MODE 2: row = i2, X offset I0 I1 row + k,               Fa = A0, Fb = A1
-// One MFMA per step of 4 k.  Lane l holds A = X(row l % 16, k = 4 s + l / 16) (FP32 widened
-// to FP64, 0 past the tensor), B = KR[4 s + l / 16][l % 16], and D rows l / 16 + 4 j, column
-// l % 16.  The T tiles x KS steps are split evenly over the nw waves (stream-K): a wave
-// stores the tiles its range covers whole straight into M and keeps the (at most two)
-// partial ones in registers for mttkrp_mfma_partials / mttkrp_mfma_reduce, which sum them
-// in wave order (a fixed order: the result does not depend on the grid).
+// A wave works on a band of 32 rows (two 16 x 16 output tiles) in steps of 4 k: per step
+// one Khatri-Rao fragment, two tensor fragments, two MFMAs.  Lane l holds A = X(row
+// l % 16 of each tile, k = 4 s + l / 16) (FP32 widened to FP64), B = KR[4 s + l / 16][l % 16]
+// (0 past K), and D rows l / 16 + 4 j, column l % 16.  Rows past the mode's size read a valid
+// row and are not stored.  The bands x KS steps are split evenly over the nw waves
+// (stream-K): a wave stores the bands its range covers whole straight into M and keeps the
+// (at most two) partial bands in registers; mttkrp_mfma_partials / mttkrp_mfma_reduce add
+// them up in wave order (a fixed order: the result does not depend on the grid).
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 constexpr int kSteps = 8;                   // k-steps whose tensor loads a lane issues together
 
 struct MfmaPart {
-  f64x4 acc0, acc1;
-  int tile0, tile1;
+  f64x4 a0, a1, b0, b1;                      // slot 0 (first partial band): tiles a0 | a1; slot 1: b0 | b1
+  int band0, band1;
 };
 
 template <int MODE>
 __device__ __forceinline__ void mttkrp_mfma(const float *X, int I0, int I1, int I2, const double *Fa,
                                             const double *Fb, double *M, int nw, MfmaPart &part) {
   const int t = tid(), lane = t & 63, wave = t >> 6;
-  part.tile0 = part.tile1 = -1;
+  part.band0 = part.band1 = -1;
   if (wave >= nw) return;
   const int Iout = MODE == 0 ? I0 : MODE == 1 ? I1 : I2;
   const int Ia = MODE == 0 ? I1 : I0, Ib = MODE == 2 ? I1 : I2;
-  const int K = Ia * Ib, KS = (K + 3) / 4, T = (Iout + 15) / 16;
+  const int K = Ia * Ib, KS = (K + 3) / 4, KF = K / 4, NB = (Iout + 31) / 32;
   const int rs = MODE == 0 ? 1 : MODE == 1 ? I0 : I0 * I1;
-  const int64_t S = (int64_t)T * KS;
+  const int64_t S = (int64_t)NB * KS;
   const int64_t s0 = (int64_t)wave * S / nw, s1 = (int64_t)(wave + 1) * S / nw;
   const int m = lane & 15, kk = lane >> 4;
+  const int inc_a = 4 % Ia, inc_b = 4 / Ia;      // k += 4 as (ia, ib) += (inc_a, inc_b), one wrap at most
   int slot = 0;
   for (int64_t s = s0; s < s1;) {
-    const int tile = (int)(s / KS);
-    const int64_t tb = (int64_t)tile * KS, te = tb + KS, e = te < s1 ? te : s1;
-    const int row = tile * 16 + m;
-    const bool rowok = row < Iout;
-    const float *xr = X + (rowok ? row : 0) * rs;
+    const int band = (int)(s / KS);
+    const int64_t tb = (int64_t)band * KS, te = tb + KS, e = te < s1 ? te : s1;
+    const int r0 = band * 32 + m, r1 = r0 + 16;
+    const float *x0 = X + (r0 < Iout ? r0 : Iout - 1) * rs;
+    const float *x1 = X + (r1 < Iout ? r1 : Iout - 1) * rs;
     int k = 4 * (int)(s - tb) + kk, ia = k % Ia, ib = k / Ia;
-    f64x4 acc = {0.0, 0.0, 0.0, 0.0};
-    int n = (int)(e - s);
-    for (; n > 0;) {
-      const int u_n = n >= kSteps ? kSteps : 1;     // a full batch of steps, or one
-      float xs[kSteps];
-      double bs[kSteps];
+    f64x4 c0 = {0.0, 0.0, 0.0, 0.0}, c1 = {0.0, 0.0, 0.0, 0.0};
+    auto offset = [&](int kx, int iax, int ibx) {
+      return MODE == 0 ? I0 * kx : MODE == 1 ? iax + I0 * I1 * ibx : kx;
+    };
+    auto advance = [&] {
+      k += 4;
+      ia += inc_a;
+      ib += inc_b;
+      if (ia >= Ia) {
+        ia -= Ia;
+        ++ib;
+      }
+    };
+    // steps [s, ef) have all four k below K; at most one checked step follows
+    const int64_t ef = tb + KF < e ? tb + KF : e;
+    int n = (int)(ef - s);
+    for (; n >= kSteps; n -= kSteps) {
+      float xa[kSteps], xb[kSteps];
+      int ias[kSteps], ibs[kSteps];
 #pragma unroll
-      for (int u = 0; u < kSteps; ++u) {
-        if (u < u_n) {
-          const bool v = k < K;
-          const int off = MODE == 0 ? I0 * k : MODE == 1 ? ia + I0 * I1 * ib : k;
-          xs[u] = (v && rowok) ? xr[off] : 0.f;
-          bs[u] = v ? Fa[ia * kBR + m] * Fb[ib * kBR + m] : 0.0;
-          k += 4;
-          ia += 4;
-          while (ia >= Ia) {
-            ia -= Ia;
-            ++ib;
-          }
-        }
+      for (int u = 0; u < kSteps; ++u) {               // all tensor loads first
+        const int off = offset(k, ia, ib);
+        xa[u] = x0[off];
+        xb[u] = x1[off];
+        ias[u] = ia;
+        ibs[u] = ib;
+        advance();
       }
 #pragma unroll
-      for (int u = 0; u < kSteps; ++u)
-        if (u < u_n) acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)xs[u], bs[u], acc, 0, 0, 0);
-      n -= u_n;
+      for (int u = 0; u < kSteps; ++u) {
+        const double bv = Fa[ias[u] * kBR + m] * Fb[ibs[u] * kBR + m];
+        c0 = __builtin_amdgcn_mfma_f64_16x16x4f64((double)xa[u], bv, c0, 0, 0, 0);
+        c1 = __builtin_amdgcn_mfma_f64_16x16x4f64((double)xb[u], bv, c1, 0, 0, 0);
+      }
+    }
+    for (int64_t j = ef - n; j < e; ++j) {
+      const bool v = k < K;
+      const int off = v ? offset(k, ia, ib) : 0;
+      const double bv = v ? Fa[ia * kBR + m] * Fb[ib * kBR + m] : 0.0;
+      const double av = v ? (double)x0[off] : 0.0, aw = v ? (double)x1[off] : 0.0;
+      c0 = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, c0, 0, 0, 0);
+      c1 = __builtin_amdgcn_mfma_f64_16x16x4f64(aw, bv, c1, 0, 0, 0);
+      advance();
     }
     if (s == tb && e == te) {
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const int r = tile * 16 + kk + 4 * j;
-        if (r < Iout) M[r * kBR + m] = acc[j];
+        const int r = band * 32 + kk + 4 * j;
+        if (r < Iout) M[r * kBR + m] = c0[j];
+        if (r + 16 < Iout) M[(r + 16) * kBR + m] = c1[j];
       }
-    } else {
-      if (slot == 0) {
-        part.acc0 = acc;
-        part.tile0 = tile;
-      } else {
-        part.acc1 = acc;
-        part.tile1 = tile;
-      }
+    } else if (slot == 0) {
+      part.a0 = c0;
+      part.a1 = c1;
+      part.band0 = band;
       slot = 1;
+    } else {
+      part.b0 = c0;
+      part.b1 = c1;
+      part.band1 = band;
     }
     s = e;
   }
 }
 
-// the partial tiles of every wave into P (slots [wave][2][16 x 16]) and their tile ids into
-// seg; P may alias the MTTKRP operands, so this follows a barrier
+// the partial bands of every wave into P ([wave][slot][32 x 16]) and their ids into seg
 __device__ __forceinline__ void mttkrp_mfma_partials(const MfmaPart &part, int nw, double *P, int *seg) {
   const int t = tid(), lane = t & 63, wave = t >> 6;
   if (wave >= nw) return;
   const int m = lane & 15, kk = lane >> 4;
   if (lane == 0) {
-    seg[2 * wave] = part.tile0;
-    seg[2 * wave + 1] = part.tile1;
+    seg[2 * wave] = part.band0;
+    seg[2 * wave + 1] = part.band1;
   }
+  double *p0 = P + (2 * wave) * 32 * kBR, *p1 = p0 + 32 * kBR;
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    if (part.tile0 >= 0) P[((2 * wave) * 16 + kk + 4 * j) * kBR + m] = part.acc0[j];
-    if (part.tile1 >= 0) P[((2 * wave + 1) * 16 + kk + 4 * j) * kBR + m] = part.acc1[j];
+    const int r = kk + 4 * j;
+    if (part.band0 >= 0) {
+      p0[r * kBR + m] = part.a0[j];
+      p0[(r + 16) * kBR + m] = part.a1[j];
+    }
+    if (part.band1 >= 0) {
+      p1[r * kBR + m] = part.b0[j];
+      p1[(r + 16) * kBR + m] = part.b1[j];
+    }
   }
 }
 
-// rows of the split tiles: the partials summed in wave order
+// rows of the split bands: the partials summed in wave order
 template <int BS>
 __device__ __forceinline__ void mttkrp_mfma_reduce(int Iout, int nw, const double *P, const int *seg, double *M) {
   for (int e = tid(); e < Iout * kBR; e += BS) {
-    const int row = e / kBR, col = e - row * kBR, tile = row >> 4, rin = row & 15;
+    const int row = e / kBR, col = e - row * kBR, band = row >> 5, rin = row & 31;
     double s = 0.0;
     bool any = false;
     for (int w = 0; w < nw; ++w)
       for (int sl = 0; sl < 2; ++sl)
-        if (seg[2 * w + sl] == tile) {
-          const double v = P[((2 * w + sl) * 16 + rin) * kBR + col];
+        if (seg[2 * w + sl] == band) {
+          const double v = P[((2 * w + sl) * 32 + rin) * kBR + col];
           s = any ? s + v : v;
           any = true;
         }
@@ -214,20 +243,19 @@ __device__ __forceinline__ void mttkrp_mfma_reduce(int Iout, int nw, const doubl
 }
 
 // one mode's MTTKRP into S.M with operands S.F[0], S.F[1]: waves [0, nw) compute, `other`
-// runs on the remaining waves between the first two barriers
+// runs on the remaining waves meanwhile; P holds 2 x 32 x 16 doubles per wave
 template <int BS, int MODE, typename Other>
-__device__ __forceinline__ void mttkrp_block(const float *X, int I0, int I1, int I2, BatchLds &S, int nw,
+__device__ __forceinline__ void mttkrp_block(const float *X, int I0, int I1, int I2, BatchLds &S, double *P, int nw,
                                              Other &&other) {
   MfmaPart part;
   if ((int)(tid() >> 6) < nw) mttkrp_mfma<MODE>(X, I0, I1, I2, S.F[0], S.F[1], S.M, nw, part);
   else {
-    part.tile0 = part.tile1 = -1;
+    part.band0 = part.band1 = -1;
     other();
   }
+  mttkrp_mfma_partials(part, nw, P, S.seg);
   __syncthreads();
-  mttkrp_mfma_partials(part, nw, &S.F[0][0], S.seg);
-  __syncthreads();
-  mttkrp_mfma_reduce<BS>(MODE == 0 ? I0 : MODE == 1 ? I1 : I2, nw, &S.F[0][0], S.seg, S.M);
+  mttkrp_mfma_reduce<BS>(MODE == 0 ? I0 : MODE == 1 ? I1 : I2, nw, P, S.seg, S.M);
   __syncthreads();
 }
 
@@ -331,7 +359,7 @@ __device__ __forceinline__ int nnls_block(int I, int R, BatchLds &S, double *Ag,
 
 // one sweep of one problem
 template <int BS, int RP>
-__device__ __forceinline__ void sweep_problem(const BatchArgs &a, int p, int sweep, BatchLds &S) {
+__device__ __forceinline__ void sweep_problem(const BatchArgs &a, int p, int sweep, BatchLds &S, double *P) {
   constexpr int NW = BS / 64;                  // the last wave solves the eigenproblem meanwhile
   const BatchDesc &d = a.desc[p];
   BatchProb &st = a.st[p];
@@ -361,9 +389,9 @@ __device__ __forceinline__ void sweep_problem(const BatchArgs &a, int p, int swe
       eig_extremes_wave<kBR>(R, S.eig, amax, amin);
       if (l == 0) glambda_betas(amax, amin, &S.prm, a.max_inner, S.betas);
     };
-    if (n == 0) mttkrp_block<BS, 0>(X, I0, I1, I2, S, NW - 1, eig);
-    else if (n == 1) mttkrp_block<BS, 1>(X, I0, I1, I2, S, NW - 1, eig);
-    else mttkrp_block<BS, 2>(X, I0, I1, I2, S, NW - 1, eig);
+    if (n == 0) mttkrp_block<BS, 0>(X, I0, I1, I2, S, P, NW - 1, eig);
+    else if (n == 1) mttkrp_block<BS, 1>(X, I0, I1, I2, S, P, NW - 1, eig);
+    else mttkrp_block<BS, 2>(X, I0, I1, I2, S, P, NW - 1, eig);
     const int In = n == 0 ? I0 : n == 1 ? I1 : I2;
     const int it = nnls_block<BS, RP>(In, R, S, n == 0 ? A0 : n == 1 ? A1 : A2, a.d1, a.d2, a.max_inner);
     if (t == 0) S.iters[n] = it;
@@ -430,7 +458,7 @@ __device__ __forceinline__ void sweep_problem(const BatchArgs &a, int p, int swe
       __syncthreads();
       gram_block<BS>(S.F[0], I0, R, S.Gh[0]);
       gram_block<BS>(S.F[1], I1, R, S.Gh[1]);
-      mttkrp_block<BS, 2>(X, I0, I1, I2, S, NW, [] {});
+      mttkrp_block<BS, 2>(X, I0, I1, I2, S, P, NW, [] {});
       double ma = 0.0, vg = 0.0;
       for (int e = t; e < I2 * kBR; e += BS) ma = fma(S.M[e], Ah2[e], ma);
       if (t < kBR * kBR) vg = (S.Gh[1][t] * S.Gh[0][t]) * S.Gh[2][t];
@@ -477,6 +505,7 @@ __device__ __forceinline__ void sweep_problem(const BatchArgs &a, int p, int swe
 template <int BS>
 __global__ __launch_bounds__(BS, 2) void k_batch_sweep(BatchArgs a) {
   __shared__ BatchLds S;
+  __shared__ double P[(BS / 64) * 2 * 32 * kBR];     // partial MTTKRP bands, per wave
   __shared__ int sp;
   const int sweep = *a.sweep_ctr;
   for (;;) {
@@ -487,10 +516,10 @@ __global__ __launch_bounds__(BS, 2) void k_batch_sweep(BatchArgs a) {
     if (q >= a.nprob) break;
     const int p = a.order[q];
     switch ((a.desc[p].R + 3) >> 2) {
-      case 1: sweep_problem<BS, 4>(a, p, sweep, S); break;
-      case 2: sweep_problem<BS, 8>(a, p, sweep, S); break;
-      case 3: sweep_problem<BS, 12>(a, p, sweep, S); break;
-      default: sweep_problem<BS, 16>(a, p, sweep, S); break;
+      case 1: sweep_problem<BS, 4>(a, p, sweep, S, P); break;
+      case 2: sweep_problem<BS, 8>(a, p, sweep, S, P); break;
+      case 3: sweep_problem<BS, 12>(a, p, sweep, S, P); break;
+      default: sweep_problem<BS, 16>(a, p, sweep, S, P); break;
     }
   }
 }
