@@ -259,49 +259,49 @@ __device__ __forceinline__ void mttkrp_block(const float *X, int I0, int I1, int
   __syncthreads();
 }
 
-// NesterovMNLS (NesterovMNLS.hpp:181-234) for the I x R entries of one factor.  Thread
-// t = (row slot rs, column c), c = t mod R, takes rows rs, rs + NS, ... (NS = BS / R).
-// Its entries of the iterate, of W~ = M + lambda A0 and of the next iterate stay in
-// registers; the extrapolation point Y sits in LDS in two buffers, so one barrier per
+// NesterovMNLS (NesterovMNLS.hpp:181-234) for the I x R entries of one factor.  The
+// gradient G = Y Zt - Wt is formed on v_mfma_f64_16x16x4f64, one 16-row tile per wave
+// (tiles w, w + NW, ...), Zt = V + lambda I held as the B fragments (lane l: Zt[4q + l/16][l%16]),
+// the A fragments read from the point Y in LDS.  Lane l owns the entries of its tiles at
+// rows l/16 + 4j, column l % 16 (the MFMA output layout): their iterate, W~ = M + lambda A0
+// and next iterate stay in registers.  Y lives in LDS in two buffers, so one barrier per
 // step both publishes the next point and ANDs the stopping test (the reference's global
-// stop); the step after a satisfied test is computed and discarded.  Per entry the
-// arithmetic is that of the single-problem kernel and the CPU restatement (four
-// interleaved partial sums, g / L through the corrected reciprocal).  Writes the result,
-// after the zero-column fallback (Cpd.hpp:206-207, an exact-zero test of each column),
-// to Ag (global rows, which hold the start point on entry) and to S.F[0]; returns the
-// step count.
+// stop); the step after a satisfied test is computed and discarded.  Per entry the update is
+// that of the single-problem kernel (g / L through the corrected reciprocal).  Writes the
+// result, after the zero-column fallback (Cpd.hpp:206-207, an exact-zero test of each
+// column), to Ag (global rows, which hold the start point on entry) and to S.F[0]; returns
+// the step count.
 template <int BS, int RP>
 __device__ __forceinline__ int nnls_block(int I, int R, BatchLds &S, double *Ag, double d1, double d2, int max_inner) {
-  constexpr int NJ = (kBMax + BS / RP - 1) / (BS / RP);   // rows per thread at the smallest NS
-  const int NS = BS / R, t = tid(), rs = t / R, c = t - rs * R;
-  const bool on = rs < NS;
+  constexpr int NW = BS / 64, TPW = (kBMax / 16 + NW - 1) / NW, NQ = RP / 4;
+  const int t = tid(), lane = t & 63, wave = t >> 6, m = lane & 15, kk = lane >> 4;
+  const int T = (I + 15) / 16;
   double *Yb0 = S.F[0], *Yb1 = S.F[1];
   const double lambda = S.prm.lambda, L = S.prm.L, invL = 1.0 / L;
-  double z[RP];
+  double zb[NQ];
 #pragma unroll
-  for (int k = 0; k < RP; ++k) {
-    const double vk = k < R ? S.V[k * kBR + c] : 0.0;
-    z[k] = k == c ? vk + lambda : vk;
+  for (int q = 0; q < NQ; ++q) {
+    const int k = 4 * q + kk;
+    zb[q] = (k < R && m < R) ? S.V[k * kBR + m] + (k == m ? lambda : 0.0) : 0.0;
   }
-  double ac[NJ], an[NJ], wt[NJ];
+  const bool on = m < R;
+  double ac[TPW][4], an[TPW][4], wt[TPW][4];
 #pragma unroll
-  for (int j = 0; j < NJ; ++j) {
-    const int i = rs + NS * j;
-    ac[j] = 0.0;
-    an[j] = 0.0;
-    wt[j] = 0.0;
-    if (on && i < I) {
-      const double a0 = Ag[i * kBR + c];
-      ac[j] = a0;
-      wt[j] = S.M[i * kBR + c] + lambda * a0;
-      Yb0[i * kBR + c] = a0;
+  for (int tt = 0; tt < TPW; ++tt) {
+    const int tile = wave + NW * tt;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int row = tile * 16 + kk + 4 * j;
+      ac[tt][j] = an[tt][j] = wt[tt][j] = 0.0;
+      if (tile < T && row < I) {
+        const double a0 = on ? Ag[row * kBR + m] : 0.0;
+        ac[tt][j] = a0;
+        wt[tt][j] = on ? S.M[row * kBR + m] + lambda * a0 : 0.0;
+        Yb0[row * kBR + m] = a0;                       // columns >= R: exact zeros in both
+        if (!on) Yb1[row * kBR + m] = 0.0;
+      }
     }
   }
-  for (int e = t; e < I * kBR; e += BS)
-    if (e % kBR >= R) {
-      Yb0[e] = 0.0;
-      Yb1[e] = 0.0;
-    }
   if (t < kBR) S.colnz[t] = 0;
   __syncthreads();
   int it = 0, b = 0;
@@ -311,48 +311,61 @@ __device__ __forceinline__ int nnls_block(int I, int R, BatchLds &S, double *Ag,
     const double beta = it < max_inner ? S.betas[it] : 0.0;
     int pass = 1;
 #pragma unroll
-    for (int j = 0; j < NJ; ++j) {
-      const int i = rs + NS * j;
-      if (on && i < I) {
-        const double *yr = Y + i * kBR;
-        double s4[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int tt = 0; tt < TPW; ++tt) {
+      const int tile = wave + NW * tt;
+      if (tile < T) {
+        f64x4 g = {0.0, 0.0, 0.0, 0.0};
+        const int ra = tile * 16 + m;                  // A fragment row (rows past I: not used)
 #pragma unroll
-        for (int k = 0; k < RP; ++k) s4[k & 3] = fma(yr[k], z[k], s4[k & 3]);
-        const double g = ((s4[0] + s4[1]) + (s4[2] + s4[3])) - wt[j];
-        const double y = yr[c];
-        if ((fabs(g * y) > d1) || (g < -d2)) pass = 0;
-        const double q0 = g * invL;
-        const double tq = y - fma(fma(-q0, L, g), invL, q0);
-        const double a = tq > 0.0 ? tq : 0.0;
-        an[j] = a;
-        Yn[i * kBR + c] = (1.0 + beta) * a - beta * ac[j];
+        for (int q = 0; q < NQ; ++q)
+          g = __builtin_amdgcn_mfma_f64_16x16x4f64(Y[(ra < kBMax ? ra : 0) * kBR + 4 * q + kk], zb[q], g, 0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int row = tile * 16 + kk + 4 * j;
+          if (on && row < I) {
+            const double gg = g[j] - wt[tt][j];
+            const double y = Y[row * kBR + m];
+            if ((fabs(gg * y) > d1) || (gg < -d2)) pass = 0;
+            const double q0 = gg * invL;
+            const double tq = y - fma(fma(-q0, L, gg), invL, q0);
+            const double av = tq > 0.0 ? tq : 0.0;
+            an[tt][j] = av;
+            Yn[row * kBR + m] = (1.0 + beta) * av - beta * ac[tt][j];
+          }
+        }
       }
     }
     const int all = __syncthreads_and(pass);
     if (all || it >= max_inner) break;
 #pragma unroll
-    for (int j = 0; j < NJ; ++j) ac[j] = an[j];
+    for (int tt = 0; tt < TPW; ++tt)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) ac[tt][j] = an[tt][j];
     b ^= 1;
     ++it;
   }
 #pragma unroll
-  for (int j = 0; j < NJ; ++j) {
-    const int i = rs + NS * j;
-    if (on && i < I && ac[j] != 0.0) S.colnz[c] = 1;
-  }
+  for (int tt = 0; tt < TPW; ++tt)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int row = (wave + NW * tt) * 16 + kk + 4 * j;
+      if (on && wave + NW * tt < T && row < I && ac[tt][j] != 0.0) S.colnz[m] = 1;
+    }
   __syncthreads();
   int fb = 0;
   for (int k = 0; k < R; ++k) fb |= S.colnz[k] == 0;
 #pragma unroll
-  for (int j = 0; j < NJ; ++j) {
-    const int i = rs + NS * j;
-    if (on && i < I) {
-      double v = ac[j];
-      if (fb) v = 0.9 * v + 0.1 * Ag[i * kBR + c];
-      Ag[i * kBR + c] = v;
-      Yb0[i * kBR + c] = v;
+  for (int tt = 0; tt < TPW; ++tt)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int row = (wave + NW * tt) * 16 + kk + 4 * j;
+      if (on && wave + NW * tt < T && row < I) {
+        double v = ac[tt][j];
+        if (fb) v = 0.9 * v + 0.1 * Ag[row * kBR + m];
+        Ag[row * kBR + m] = v;
+        Yb0[row * kBR + m] = v;
+      }
     }
-  }
   __syncthreads();
   return it;
 }
