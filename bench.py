@@ -157,7 +157,7 @@ def batch_problems():
     return [tuple(int(v) for v in d) for d in dims], [int(r) for r in ranks]
 
 
-def batch_cpu_baseline(dims, ranks, sweeps=4, every=2):
+def batch_cpu_baseline(dims, ranks, sweeps=10, every=1):
     """The CPU oracle's cpd() (standard MTTKRP, OpenMP over output rows, Gram-form cost)
     on every `every`-th problem of the batch, one problem after another, on the FP32-
     rounded tensors the device holds; scaled to whole-batch sweeps per second."""
