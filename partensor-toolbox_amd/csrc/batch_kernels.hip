@@ -516,7 +516,7 @@ __device__ __forceinline__ void sweep_problem(const BatchArgs &a, int p, int swe
 }
 
 template <int BS>
-__global__ __launch_bounds__(BS, 2) void k_batch_sweep(BatchArgs a) {
+__global__ __launch_bounds__(BS, BS == 512 ? 2 : 1) void k_batch_sweep(BatchArgs a) {
   __shared__ BatchLds S;
   __shared__ double P[(BS / 64) * 2 * 32 * kBR];     // partial MTTKRP bands, per wave
   __shared__ int sp;

@@ -41,7 +41,9 @@ struct ptn_handle_s {
   int device = 0;
   hipStream_t stream = nullptr;
   hipStream_t side = nullptr;                // eigen / momentum set-up of the next NNLS, beside the contractions
-  hipEvent_t ev[2] = {nullptr, nullptr};     // [0] main -> side, [1] side -> main
+  hipStream_t aux = nullptr;                 // the order-3 tree's right node, beside mode 1
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};   // [0] main -> side, [1] side -> main,
+                                                             // [2] main -> aux, [3] aux -> main
   std::string err;
   int N = 0;
   int64_t dims[kMaxOrder] = {0};
@@ -264,7 +266,7 @@ struct Work {
   bool f32;                                  // FP32 tensor stream (arith F32 / BF16X3 / F32_F64)
   bool mfma32;                               // contractions on FP32-output kernels (F32 / BF16X3)
   DevBuf<double> fac, old, nrm, acc, grams, agr, Vm, Vacc, M, Aold, colsum, part, T, YL, nscr, scal, hist, betas,
-      gpart;
+      gpart, cscr;
   DevBuf<float> T32, YL32, YL2, part32, wf;
   DevBuf<uint16_t> whi, wlo;
   DevBuf<NnlsParams> prm;
@@ -279,6 +281,7 @@ struct Work {
     int64_t sel_off = 0;
   } left, right;                             // dimension-tree nodes cached between the modes of a sweep
   bool left_ready = false;                   // left node already formed for the coming mode 0 (line-search pass)
+  bool right_ready = false;                  // order-3 right node X x_0 A_0 already formed (aux stream)
 
   Work(ptn_handle hh, int R_, int arith_, bool need_tree)
       : h(hh), N(hh->N), R(R_), ldr(round_up16(R_)), rpf((R_ + 31) / 32 * 32), arith(arith_) {
@@ -301,6 +304,7 @@ struct Work {
     prm.alloc(h, kMaxOrder + 1); mask.alloc(h, nnls_mask_words(maxI)); flags.alloc(h, 4);
     nscr.alloc(h, 2 * kMaxOrder * kMaxRank); scal.alloc(h, 8); betas.alloc(h, (size_t)(N + 1) * (kMaxInner + 1));
     gpart.alloc(h, (size_t)((maxI + 31) / 32) * R * R);
+    cscr.alloc(h, (size_t)3 * R * R + R);    // R x R work of the unconstrained / orthogonality updates
     const int64_t C = total / dims[0];
     const int64_t yl = dims[0] * dims[1];
     // reduction partials: mode-0 split-K (<= 256 splits of I_0 rows) and row contractions (<= 64 splits)
@@ -337,7 +341,7 @@ struct Work {
 
   // Khatri-Rao operand of an FP32 contraction, packed in fragment order; with k2 the
   // operand carries both (2 rpf columns: k in the lower half, k2 in the upper).
-  void pack(const KrSpec &k, int64_t K, int kperm, const KrSpec *k2 = nullptr) {
+  void pack(const KrSpec &k, int64_t K, int kperm, const KrSpec *k2 = nullptr, hipStream_t ps = nullptr) {
     const int width = k2 ? 2 * rpf : rpf;
     const int64_t frag = wpack_elems(K, width);
     if (frag > wcap) {
@@ -350,8 +354,8 @@ struct Work {
       wcap = frag;
     }
     const bool x3 = arith == PTN_ARITH_BF16X3;
-    pack_w(h->stream, k, k2, K, width, ldr, kperm, x3 ? (void *)whi.p : nullptr, x3 ? (void *)wlo.p : nullptr,
-           x3 ? nullptr : wf.p);
+    pack_w(ps ? ps : h->stream, k, k2, K, width, ldr, kperm, x3 ? (void *)whi.p : nullptr,
+           x3 ? (void *)wlo.p : nullptr, x3 ? nullptr : wf.p);
   }
 };
 
@@ -407,14 +411,14 @@ Node node_cm(Work &w, const KrSpec &k, int64_t Mrows, int64_t K, int S, double *
 }
 
 // T[c][r] = sum_{a < L} X[a + L c] W(a)[r]
-Node node_ck(Work &w, const KrSpec &k, int64_t L, int64_t C) {
+Node node_ck(Work &w, const KrSpec &k, int64_t L, int64_t C, hipStream_t s = nullptr) {
   ptn_handle h = w.h;
-  hipStream_t s = h->stream;
+  if (!s) s = h->stream;
   if (!w.mfma32) {
     gemm_ck_f64(s, w.x64(), w.x32(), L, C, k, w.ldr, w.T);
     return Node{w.T.p, nullptr, w.ldr};
   }
-  w.pack(k, L, 1);
+  w.pack(k, L, 1, nullptr, s);
   gemm_ck_f32(s, w.gmode(), w.x32(), L, C, w.rpf, w.whi, w.wlo, w.wf, w.T32, 0, nullptr, w.ldr, nullptr);
   return Node{nullptr, w.T32.p, w.rpf};
 }
@@ -508,6 +512,13 @@ void mttkrp_tree(Work &w, double *base, int mode, double *M) {
       contract_node(w, node(w.left), kr_of(w, base, {0, 1}), mode, M);
       return;
     }
+    if (w.right_ready) {                     // formed on the aux stream after mode 0 (right_node_early)
+      w.right_ready = false;
+      ptn_handle h = w.h;
+      HIPCHK(hipStreamWaitEvent(h->stream, h->ev[3], 0));
+      contract_node(w, node(w.right), kr_of(w, base, {1, 2}), 1, M);
+      return;
+    }
     if (w.mfma32) {
       right_node_fused(w, base, M);
       return;
@@ -524,6 +535,21 @@ void mttkrp_tree(Work &w, double *base, int mode, double *M) {
   if (mode == 2) keep(w.right, node_ck(w, kr_of(w, base, {0, 1}), d[0] * d[1], d[2] * d[3]));
   if (mode <= 1) contract_node(w, node(w.left), kr_of(w, base, {0, 1}), mode, M);
   else contract_node(w, node(w.right), kr_of(w, base, {2, 3}), mode - 2, M);
+}
+
+// The order-3 tree's right node X x_0 A_0 (R x I1 x I2, FP32) depends only on A_0, so in
+// a sweep it is formed on the aux stream as soon as mode 0 is updated, beside mode 1's
+// contraction, eigen set-up and Nesterov solve; mode 2 then contracts it with A_1
+// (DimTrees.hpp:668-800: the same node, formed earlier).  The main stream waits for it
+// at mode 2.
+void right_node_early(Work &w, double *base) {
+  ptn_handle h = w.h;
+  HIPCHK(hipEventRecord(h->ev[2], h->stream));
+  HIPCHK(hipStreamWaitEvent(h->aux, h->ev[2], 0));
+  const Node nd = node_ck(w, kr_of(w, base, {0}), w.dims[0], w.dims[1] * w.dims[2], h->aux);
+  w.right.p64 = nd.p64; w.right.p32 = nd.p32; w.right.ld = nd.ld; w.right.sel = nullptr; w.right.sel_off = 0;
+  HIPCHK(hipEventRecord(h->ev[3], h->aux));
+  w.right_ready = true;
 }
 
 // Line-search pass fused with the next sweep's left node: one pass over X forms
@@ -692,6 +718,34 @@ void update_nonneg(Work &w, int mode, const ptn_options &o, const ModeIn &in) {
   }
 }
 
+// update_factor, Cpd.hpp:189-228: the constraint of the mode picks the update
+void update_factor(Work &w, int mode, const ptn_options &o, const ModeIn &in) {
+  ptn_handle h = w.h;
+  hipStream_t s = h->stream;
+  const int64_t I = w.dims[mode];
+  const int R = w.R;
+  double *A = w.F(mode);
+  double *c0 = w.cscr.p, *c1 = c0 + (int64_t)R * R, *c2 = c1 + (int64_t)R * R;
+  switch (o.constraints[mode]) {
+    case PTN_NONNEGATIVITY:
+      update_nonneg(w, mode, o, in);
+      return;
+    case PTN_UNCONSTRAINED:                   // A = M V^{-1} (Cpd.hpp:196-200)
+      lu_inverse(s, R, in.V, c0);
+      apply_right(s, I, R, w.ldr, w.M, c0, A);
+      break;
+    case PTN_ORTHOGONALITY:                   // A = M (M^T M)^{-1/2} (Cpd.hpp:210-219)
+      gram_of(w, w.M, I, c0);
+      jacobi_eig(s, R, c0, c2, c1);
+      inv_sqrt_matrix(s, R, c1, c2, c0);
+      apply_right(s, I, R, w.ldr, w.M, c0, A);
+      break;
+    default:                                  // sparsity / constant: the factor is left as it is
+      break;
+  }
+  params_no_nnls(s, in.p);
+}
+
 }  // namespace
 
 // ======================================================================================
@@ -732,6 +786,7 @@ ptn_status ptn_create(ptn_handle *out, int device) {
     HIPCHK(hipSetDevice(device));
     HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     HIPCHK(hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&h->aux, hipStreamNonBlocking));
     for (hipEvent_t &e : h->ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HIPCHK(hipMalloc((void **)&h->scal, 16 * sizeof(double)));
     HIPCHK(hipMalloc((void **)&h->part, kNormParts * sizeof(double)));
@@ -749,6 +804,7 @@ void ptn_destroy(ptn_handle h) {
   (void)hipSetDevice(h->device);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   if (h->side) (void)hipStreamSynchronize(h->side);
+  if (h->aux) (void)hipStreamSynchronize(h->aux);
   if (h->x64) (void)hipFree(h->x64);
   if (h->x32) (void)hipFree(h->x32);
   delete h->batch;
@@ -757,6 +813,7 @@ void ptn_destroy(ptn_handle h) {
   if (h->part) (void)hipFree(h->part);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   if (h->side) (void)hipStreamDestroy(h->side);
+  if (h->aux) (void)hipStreamDestroy(h->aux);
   for (hipEvent_t e : h->ev)
     if (e) (void)hipEventDestroy(e);
   delete h;
@@ -900,7 +957,8 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
     REQUIRE(o.mttkrp_form == PTN_MTTKRP_STANDARD || tree, PTN_ERR_INVALID, "bad mttkrp_form");
     REQUIRE(!tree || h->N == 3 || h->N == 4, PTN_ERR_UNSUPPORTED, "dimension tree built for N = 3, 4");
     for (int k = 0; k < h->N; ++k)
-      REQUIRE(o.constraints[k] == PTN_NONNEGATIVITY, PTN_ERR_UNSUPPORTED, "only the nonnegativity constraint is built");
+      REQUIRE(o.constraints[k] >= PTN_UNCONSTRAINED && o.constraints[k] <= PTN_CONSTANT, PTN_ERR_INVALID,
+              "bad constraint");
     const bool pair = o.fuse_accel != 0;
     REQUIRE(!pair || (tree && (o.arith == PTN_ARITH_F32 || o.arith == PTN_ARITH_BF16X3)), PTN_ERR_UNSUPPORTED,
             "fuse_accel needs the dimension tree with F32 or BF16X3 contractions");
@@ -986,8 +1044,9 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
         if (tree) mttkrp_tree(w, w.fac, n, w.M);
         else mttkrp_standard(w, w.fac, n, w.M);
         join();
-        update_nonneg(w, n, o, sel[n]);
+        update_factor(w, n, o, sel[n]);
         gram_of(w, w.F(n), w.dims[n], w.G(n));
+        if (n == 0 && tree && w.mfma32 && N == 3) right_node_early(w, w.fac);
         if (n == last)
           cost_gram(ms, w.dims[last] * w.ldr, w.M, w.F(last), R, sel[last].V, w.G(last), w.scal.p, w.scal.p + 1, 0);
       }
@@ -1109,6 +1168,7 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
     }
     HIPCHK(hipStreamSynchronize(h->stream));
     HIPCHK(hipStreamSynchronize(h->side));
+    HIPCHK(hipStreamSynchronize(h->aux));
     r.sweep_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     HIPCHK(hipGetLastError());
     download_factors(w, snaps[final_it & 1], out_packed);

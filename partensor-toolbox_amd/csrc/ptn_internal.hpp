@@ -85,6 +85,13 @@ void batch_init(hipStream_t s, const BatchArgs &a);
 void batch_sweep(hipStream_t s, const BatchArgs &a, int block, int grid);
 void batch_next_sweep(hipStream_t s, int *ctr);
 
+// ---- update_factor's unconstrained and orthogonality branches (constraint_kernels.hip) ----
+void lu_inverse(hipStream_t s, int R, const double *V, double *Vi);            // NaN when singular
+void jacobi_eig(hipStream_t s, int R, const double *A, double *evals, double *Q);
+void inv_sqrt_matrix(hipStream_t s, int R, const double *Q, const double *ev, double *T);
+void apply_right(hipStream_t s, int64_t I, int R, int ldr, const double *M, const double *T, double *A);
+void params_no_nnls(hipStream_t s, NnlsParams *p);
+
 // ---- the NNLS warm-start network (mlp_kernels.hip) ----
 // 2208 -> 256 -> 256 -> 64 with ReLU: inputs vech(V) (64 x 65 / 2) + MTTKRP row + previous row.
 constexpr int kMlpR = 64, kMlpVech = kMlpR * (kMlpR + 1) / 2, kMlpIn = kMlpVech + 2 * kMlpR, kMlpH = 256;

@@ -189,15 +189,30 @@ def test_order4_gemm_node_full_size_errors(h):
 
 # --------------------------------------------------------------------- ALS --
 
-def _compare_runs(Fg, stg, logg, Fo, sto, logo, tol):
+def _compare_runs(Fg, stg, logg, Fo, sto, logo, tol, final=True):
     assert stg["ao_iter"] == sto["ao_iter"]
     assert [l["nnls_iters"] for l in logg] == [l["nnls_iters"] for l in logo]
     assert [l["accel_accepted"] for l in logg] == [l["accel_accepted"] for l in logo]
     assert [l["zero_col_fallback"] for l in logg] == [l["zero_col_fallback"] for l in logo]
+    for a, b in zip(logg, logo):
+        assert a["f_value"] == pytest.approx(b["f_value"], rel=tol)
+    if not final:
+        return
     assert (stg["accel_coeff"], stg["accel_fail"]) == (sto["accel_coeff"], sto["accel_fail"])
     assert stg["f_value"] == pytest.approx(sto["f_value"], rel=tol)
     for a, b in zip(Fg, Fo):
         assert P.rel_err(a, b) < tol
+
+
+def _first_accel_tie(logg, logo):
+    """The first sweep whose line-search outcome differs, if the oracle's two costs tie
+    there to 1e-9 (costs carry ~1e-14 of summation-order difference, so such a test may go
+    either way); None when the outcomes agree throughout."""
+    for k, (a, b) in enumerate(zip(logg, logo)):
+        if a["accel_accepted"] != b["accel_accepted"]:
+            assert abs(b["f_value"] - b["f_accel"]) <= 1e-9 * b["f_value"], (k, b["f_value"], b["f_accel"])
+            return k
+    return None
 
 
 @pytest.mark.parametrize("form", [P.MTTKRP_STANDARD, P.MTTKRP_DIMTREE])
@@ -327,6 +342,36 @@ def test_als_higher_order_standard_form(h, dims, R, seed, sweeps):
         h.cpd(R, init, mttkrp_form=P.MTTKRP_DIMTREE, **kw)
 
 
+@pytest.mark.parametrize("cons", [[0, 0, 0], [2, 1, 1], [1, 0, 2], [1, 4, 1], [0, 2, 1, 3]])
+def test_als_other_constraints(h, cons):
+    """update_factor's other branches on the device (Cpd.hpp:189-228): unconstrained
+    M V^-1 (LU with partial pivoting), orthogonality M (M^T M)^-1/2 (round-robin Jacobi),
+    sparsity / constant (factor not updated), mixed with nonnegativity, both MTTKRP forms,
+    against the oracle's update_factor; Normalize takes the first non-orthogonal factor."""
+    dims = [30, 25, 20, 12][:len(cons)]
+    R = 4
+    say("constraints", cons)
+    X, _ = o.make_tensor(dims, R, seed=13, snr_db=30.0)
+    init = o.gen_factors(dims, R, seed=14)
+    h.set_tensor(X)
+    kw = dict(max_iter=10, threshold_error=0.0, constraints=cons)
+    for form, oform in ((P.MTTKRP_STANDARD, o.MTTKRP_STANDARD), (P.MTTKRP_DIMTREE, o.MTTKRP_DIMTREE)):
+        Fg, stg, logg = h.cpd(R, init, mttkrp_form=form, **kw)
+        Fo, sto, logo = o.cpd(X, R, init, mttkrp_form=oform, cost_form=o.COST_GRAM, **kw)
+        tie = _first_accel_tie(logg, logo)
+        if tie is None:
+            _compare_runs(Fg, stg, logg, Fo, sto, logo, 1e-10)
+        else:
+            say("line-search tie at sweep", tie + 1, "- compared up to there")
+            _compare_runs(Fg, stg, logg[:tie], Fo, sto, logo[:tie], 1e-10, final=False)
+        for k, c in enumerate(cons):
+            if c == 2:                                      # orthonormal columns
+                assert np.allclose(Fg[k].T @ Fg[k], np.eye(R), atol=1e-10)
+            if c == 4:                                      # constant: only rescaled (Normalize)
+                ratio = Fg[k] / init[k]
+                assert np.allclose(ratio, ratio[0:1, :], rtol=1e-12)
+
+
 def test_zero_column_fallback_on_gpu(h):
     say("zero-column fallback")
     rng = np.random.default_rng(12)
@@ -347,6 +392,8 @@ def test_invalid_arguments_raise(h):
     init = [np.ones((I, 3)) for I in (4, 5, 6)]
     with pytest.raises(P.PtnError):
         h.cpd(65, [np.ones((I, 65)) for I in (4, 5, 6)])          # rank above the wave-per-row limit
+    with pytest.raises(P.PtnError):
+        h.cpd(3, init, constraints=[1, 7, 1])                       # no such constraint
     with pytest.raises(P.PtnError):
         h.mttkrp(init, 5)                                           # bad mode
     with pytest.raises(P.PtnError):
