@@ -306,16 +306,16 @@ struct Work {
     gpart.alloc(h, (size_t)((maxI + 31) / 32) * R * R);
     cscr.alloc(h, (size_t)3 * R * R + R);    // R x R work of the unconstrained / orthogonality updates
     const int64_t C = total / dims[0];
-    const int64_t yl = dims[0] * dims[1];
+    const int64_t yl = left_rows();
     // reduction partials: mode-0 split-K (<= 256 splits of I_0 rows) and row contractions (<= 64 splits)
     part_cap = std::max<int64_t>(256 * dims[0] * ldr, 64 * maxI * ldr);
     if (!mfma32) {
-      if (need_tree && N == 4) part_cap = std::max<int64_t>(part_cap, 64 * yl * ldr);
+      if (need_tree && N >= 4) part_cap = std::max<int64_t>(part_cap, 64 * yl * ldr);
       T.alloc(h, C * ldr);
       if (need_tree) YL.alloc(h, yl * ldr);
     } else {
       part32_cap = 256 * dims[0] * rpf;
-      if (need_tree && N == 4) part32_cap = std::max<int64_t>(part32_cap, 64 * yl * rpf);
+      if (need_tree && N >= 4) part32_cap = std::max<int64_t>(part32_cap, 64 * yl * rpf);
       part32.alloc(h, part32_cap);
       T32.alloc(h, std::max(C, dims[0]) * rpf);
       if (need_tree) YL32.alloc(h, yl * rpf);
@@ -324,13 +324,21 @@ struct Work {
   }
   // buffers of the paired left-node pass (dimension tree, FP32-output contractions)
   void enable_pair() {
-    const int64_t yl = dims[0] * dims[1];
+    const int64_t yl = left_rows();
     YL2.alloc(h, yl * 2 * rpf);
-    if (N == 4 && 64 * yl * 2 * rpf > part32_cap) {
+    if (N >= 4 && 64 * yl * 2 * rpf > part32_cap) {
       part32.~DevBuf(); new (&part32) DevBuf<float>();
       part32_cap = 64 * yl * 2 * rpf;
       part32.alloc(h, part32_cap);
     }
+  }
+  // the tree's root splits at half = ceil(N / 2) (DIM_HALF_SIZE, DimTrees.hpp:274): the left
+  // node holds modes [0, half), the right node modes [half, N)
+  int half() const { return (N + 1) / 2; }
+  int64_t left_rows() const {
+    int64_t r = 1;
+    for (int k = 0; k < half(); ++k) r *= dims[k];
+    return r;
   }
   double *F(int k) { return fac.p + off[k]; }
   double *Fb(double *base, int k) { return base + off[k]; }
@@ -380,6 +388,17 @@ int pick_splits(int64_t blocks_per_split, int64_t work_per_split_unit, int64_t m
 KrSpec kr_of(Work &w, double *base, std::initializer_list<int> modes) {
   KrSpec k{};
   for (int m : modes) {
+    k.dim[k.nm] = w.dims[m];
+    k.fac[k.nm] = w.Fb(base, m);
+    k.nm++;
+  }
+  return k;
+}
+
+// the Khatri-Rao operand of modes [lo, hi)
+KrSpec kr_range(Work &w, double *base, int lo, int hi) {
+  KrSpec k{};
+  for (int m = lo; m < hi; ++m) {
     k.dim[k.nm] = w.dims[m];
     k.fac[k.nm] = w.Fb(base, m);
     k.nm++;
@@ -527,14 +546,19 @@ void mttkrp_tree(Work &w, double *base, int mode, double *M) {
     contract_node(w, node(w.right), kr_of(w, base, {1, 2}), 1, M);
     return;
   }
+  // order >= 4: the root's two children, each one pass over X (the GEMM node with the
+  // Khatri-Rao rows formed on the fly, so the root-to-child TTV chain is fused); every leaf
+  // contracts its child over the child's other modes
+  const int hf = w.half();
+  const int64_t Ml = w.left_rows(), Kl = w.total / Ml;
   if (mode == 0 && !w.left_ready) {
-    const int S = pick_splits((d[0] * d[1] + 255) / 256, d[2] * d[3] / 256, 64);
-    keep(w.left, node_cm(w, kr_of(w, base, {2, 3}), d[0] * d[1], d[2] * d[3], S, w.YL.p, w.YL32.p));
+    const int S = pick_splits((Ml + 255) / 256, Kl / 256, 64);
+    keep(w.left, node_cm(w, kr_range(w, base, hf, N), Ml, Kl, S, w.YL.p, w.YL32.p));
   }
   w.left_ready = false;
-  if (mode == 2) keep(w.right, node_ck(w, kr_of(w, base, {0, 1}), d[0] * d[1], d[2] * d[3]));
-  if (mode <= 1) contract_node(w, node(w.left), kr_of(w, base, {0, 1}), mode, M);
-  else contract_node(w, node(w.right), kr_of(w, base, {2, 3}), mode - 2, M);
+  if (mode == hf) keep(w.right, node_ck(w, kr_range(w, base, 0, hf), Ml, Kl));
+  if (mode < hf) contract_node(w, node(w.left), kr_range(w, base, 0, hf), mode, M);
+  else contract_node(w, node(w.right), kr_range(w, base, hf, N), mode - hf, M);
 }
 
 // The order-3 tree's right node X x_0 A_0 (R x I1 x I2, FP32) depends only on A_0, so in
@@ -561,11 +585,11 @@ void right_node_early(Work &w, double *base) {
 void left_node_pair(Work &w, double *cur, double *acc, double *Mhat) {
   ptn_handle h = w.h;
   hipStream_t s = h->stream;
-  const int64_t *d = w.dims;
   const bool n3 = w.N == 3;
-  const KrSpec k1 = n3 ? kr_of(w, cur, {2}) : kr_of(w, cur, {2, 3});
-  const KrSpec k2 = n3 ? kr_of(w, acc, {2}) : kr_of(w, acc, {2, 3});
-  const int64_t Mrows = d[0] * d[1], K = w.total / Mrows;
+  const int hf = w.half();
+  const KrSpec k1 = kr_range(w, cur, hf, w.N);
+  const KrSpec k2 = kr_range(w, acc, hf, w.N);
+  const int64_t Mrows = w.left_rows(), K = w.total / Mrows;
   const int width = 2 * w.rpf;
   w.pack(k1, K, 0, &k2);
   const int S = n3 ? 1 : pick_splits((Mrows + 255) / 256, K / 256, 64);
@@ -576,7 +600,7 @@ void left_node_pair(Work &w, double *cur, double *acc, double *Mhat) {
     gemm_cm_f32(s, w.gmode(), w.x32(), Mrows, K, width, S, w.whi, w.wlo, w.wf, w.part32);
     reduce_splits_f32(s, w.part32, S, Mrows * width, w.YL2);
   }
-  contract_node(w, Node{nullptr, w.YL2.p + w.rpf, width}, kr_of(w, acc, {0, 1}), 0, Mhat);
+  contract_node(w, Node{nullptr, w.YL2.p + w.rpf, width}, kr_range(w, acc, 0, hf), 0, Mhat);
 }
 
 // Hadamard order of V_n: ascending (PartialCwiseProd, right-nested) for the standard
@@ -955,13 +979,14 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
     REQUIRE(o.accel_coeff >= 1, PTN_ERR_INVALID, "accel_coeff must be >= 1");
     const bool tree = o.mttkrp_form == PTN_MTTKRP_DIMTREE;
     REQUIRE(o.mttkrp_form == PTN_MTTKRP_STANDARD || tree, PTN_ERR_INVALID, "bad mttkrp_form");
-    REQUIRE(!tree || h->N == 3 || h->N == 4, PTN_ERR_UNSUPPORTED, "dimension tree built for N = 3, 4");
+    REQUIRE(!tree || h->N <= kMaxOrder, PTN_ERR_UNSUPPORTED, "dimension tree built for N = 3 .. 8");
     for (int k = 0; k < h->N; ++k)
       REQUIRE(o.constraints[k] >= PTN_UNCONSTRAINED && o.constraints[k] <= PTN_CONSTANT, PTN_ERR_INVALID,
               "bad constraint");
     const bool pair = o.fuse_accel != 0;
     REQUIRE(!pair || (tree && (o.arith == PTN_ARITH_F32 || o.arith == PTN_ARITH_BF16X3)), PTN_ERR_UNSUPPORTED,
             "fuse_accel needs the dimension tree with F32 or BF16X3 contractions");
+    REQUIRE(!pair || h->N <= 4, PTN_ERR_UNSUPPORTED, "fuse_accel is built for orders 3 and 4");
     REQUIRE(!o.warm_start || (h->mlp && h->mlp->ready), PTN_ERR_STATE,
             "warm_start needs a network (ptn_mlp_init or ptn_mlp_set_params)");
 
@@ -1197,7 +1222,7 @@ ptn_status ptn_mttkrp_dimtree(ptn_handle h, int R, const double *factors_packed,
   return run(h, [&] {
     check_arith(h, arith);
     REQUIRE(R >= 1 && R <= PTN_MAX_RANK, PTN_ERR_UNSUPPORTED, "rank must be in [1, 64]");
-    REQUIRE(h->N == 3 || h->N == 4, PTN_ERR_UNSUPPORTED, "dimension tree built for N = 3, 4");
+    REQUIRE(h->N >= 3 && h->N <= kMaxOrder, PTN_ERR_UNSUPPORTED, "dimension tree built for N = 3 .. 8");
     REQUIRE(factors_packed && M_packed_out, PTN_ERR_INVALID, "null argument");
     Work w(h, R, arith, true);
     upload_factors(w, factors_packed, w.fac);

@@ -326,10 +326,10 @@ def test_als_fp32_stream_and_fused_line_search(h, arith, dims, R):
 @pytest.mark.parametrize("dims,R,seed,sweeps", [([5, 4, 6, 3, 4], 3, 5, 9), ([4, 3, 5, 3, 4, 2], 3, 6, 10),
                                                 ([3, 4, 5, 3, 2, 3], 2, 7, 10)])
 def test_als_higher_order_standard_form(h, dims, R, seed, sweeps):
-    """Orders 5 and 6 through the per-mode form (the tree is built for 3 and 4).  The
-    sweep counts stop before the runs converge to where f and the accelerated cost tie
-    to ~1e-14 and the decision rests on the last bits (smallest oracle margin here
-    >= 1.4e-4 relative)."""
+    """Orders 5 and 6 through the per-mode form.  The sweep counts stop before the runs
+    converge to where f and the accelerated cost tie to ~1e-14 and the decision rests on
+    the last bits (smallest oracle margin here >= 1.4e-4 relative).  The fused line-search
+    pass is built for orders 3 and 4 only."""
     say("ALS order", len(dims), dims, R)
     X, _ = o.make_tensor(dims, R, seed=seed, snr_db=25.0)
     init = o.gen_factors(dims, R, seed=seed + 1)
@@ -339,7 +339,7 @@ def test_als_higher_order_standard_form(h, dims, R, seed, sweeps):
     Fo, sto, logo = o.cpd(X, R, init, mttkrp_form=o.MTTKRP_STANDARD, cost_form=o.COST_GRAM, **kw)
     _compare_runs(Fg, stg, logg, Fo, sto, logo, 1e-10)
     with pytest.raises(P.PtnError):
-        h.cpd(R, init, mttkrp_form=P.MTTKRP_DIMTREE, **kw)
+        h.cpd(R, init, mttkrp_form=P.MTTKRP_DIMTREE, arith=P.ARITH_F32, fuse_accel=1, **kw)
 
 
 @pytest.mark.parametrize("cons", [[0, 0, 0], [2, 1, 1], [1, 0, 2], [1, 4, 1], [0, 2, 1, 3]])
@@ -370,6 +370,37 @@ def test_als_other_constraints(h, cons):
             if c == 4:                                      # constant: only rescaled (Normalize)
                 ratio = Fg[k] / init[k]
                 assert np.allclose(ratio, ratio[0:1, :], rtol=1e-12)
+
+
+@pytest.mark.parametrize("dims,R", [([6, 5, 7, 4, 5], 3), ([4, 5, 3, 4, 3, 4], 3), ([3, 4, 3, 2, 3, 2, 3], 2),
+                                    ([3, 2, 3, 2, 2, 3, 2, 3], 2)])
+def test_dimtree_higher_orders(h, dims, R):
+    """Orders 5-8 in the tree: the root's children (modes [0, ceil(N/2)) and the rest), each
+    one pass over X; every MTTKRP against the oracle's, FP64 and FP32 stream; then ALS in the
+    tree form against the oracle's standard form."""
+    say("tree order", len(dims), dims)
+    rng = np.random.default_rng(len(dims))
+    X = np.asfortranarray(rng.standard_normal(dims))
+    A = [rng.random((I, R)) for I in dims]
+    h.set_tensor(X, storage=P.TENSOR_BOTH)
+    D = h.mttkrp_dimtree(A)
+    D32 = h.mttkrp_dimtree(A, arith=P.ARITH_F32_F64)
+    for mode in range(len(dims)):
+        ref = o.mttkrp(X, A, mode)
+        assert P.rel_err(D[mode], ref) < 1e-13, mode
+        assert P.rel_err(D32[mode], ref) < 1e-6, mode
+    Xc, _ = o.make_tensor(dims, R, seed=len(dims) + 20, snr_db=30.0)
+    init = o.gen_factors(dims, R, seed=len(dims) + 21)
+    h.set_tensor(Xc)
+    kw = dict(max_iter=8, threshold_error=0.0)
+    Fg, stg, logg = h.cpd(R, init, mttkrp_form=P.MTTKRP_DIMTREE, **kw)
+    Fo, sto, logo = o.cpd(Xc, R, init, cost_form=o.COST_GRAM, **kw)
+    tie = _first_accel_tie(logg, logo)
+    if tie is None:
+        _compare_runs(Fg, stg, logg, Fo, sto, logo, 1e-9)
+    else:
+        say("line-search tie at sweep", tie + 1, "- compared up to there")
+        _compare_runs(Fg, stg, logg[:tie], Fo, sto, logo[:tie], 1e-9, final=False)
 
 
 def test_zero_column_fallback_on_gpu(h):
