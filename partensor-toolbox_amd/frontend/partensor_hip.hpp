@@ -396,7 +396,7 @@ HipStatus<hip::Tensor<(int)N>> cpd(execution::hip_policy, const std::array<int, 
   return hip::detail::dispatch<hip::Tensor<(int)N>>(X, R, options, &init, false);
 }
 
-// ---- cpdDimTree(execution::hip, ...) : CpdDimTree.hpp overload set (orders 3 and 4) ----
+// ---- cpdDimTree(execution::hip, ...) : CpdDimTree.hpp overload set (orders 3 to 8) ----
 template <typename Tensor_>
 HipStatus<Tensor_> cpdDimTree(execution::hip_policy, const Tensor_ &tnsX, std::size_t R) {
   return hip::detail::dispatch<Tensor_>(tnsX, R, HipOptions<Tensor_>(), nullptr, true);
