@@ -1088,13 +1088,13 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
       const bool cap = it >= (int)o.max_iter;
       bool tried = false;
       if (!cap && o.acceleration) {
-        HIPCHK(hipMemcpyAsync(w.nrm, w.fac, w.fsz * sizeof(double), hipMemcpyDeviceToDevice, ms));
         if (it > 1) {
           // line_search_accel, Cpd.hpp:247-278: step (k+1)^(1/accel_coeff); the device's
           // coefficient is c_known or, after one more fifth failure, c_known + 1
           tried = true;
           const double step_c = std::pow((double)(it + 1), 1.0 / (double)c_known);
           const double step_c1 = std::pow((double)(it + 1), 1.0 / (double)(c_known + 1));
+          // A_hat = A_old + step (A - A_old); A_old = A (the post-Normalize factors) in the same pass
           axpy_accel(ms, w.fsz, step_c, step_c1, c_known, w.flags.p + 3, w.old, w.fac, w.acc);
           for (int k = 0; k < N; ++k) gram_of(w, w.Fb(w.acc, k), w.dims[k], w.agr.p + (int64_t)k * R * R);
           // accel_cost_function (CpdDimTree.hpp:174-181): its fit term from the last mode's
@@ -1141,7 +1141,9 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
             prepped0 = true;
           }
         }
-        HIPCHK(hipMemcpyAsync(w.old, w.nrm, w.fsz * sizeof(double), hipMemcpyDeviceToDevice, ms));
+        else {
+          HIPCHK(hipMemcpyAsync(w.old, w.fac, w.fsz * sizeof(double), hipMemcpyDeviceToDevice, ms));
+        }
       }
       HIPCHK(hipMemcpyAsync(snaps[it & 1], w.fac, w.fsz * sizeof(double), hipMemcpyDeviceToDevice, ms));
       HIPCHK(hipEventRecord(evs.e[it & 1], ms));

@@ -219,7 +219,7 @@ struct AccelCopy {
   NnlsParams *prm_dst;
 };
 void accel_decide(hipStream_t s, const double *scal, double thr, int *flags, const AccelCopy &cp);
-void axpy_accel(hipStream_t s, int64_t n, double step_c, double step_c1, int c, const int *coeff, const double *old,
+void axpy_accel(hipStream_t s, int64_t n, double step_c, double step_c1, int c, const int *coeff, double *old,
                 const double *cur, double *out);
 
 }  // namespace ptn

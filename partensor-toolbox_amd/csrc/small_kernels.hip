@@ -388,11 +388,15 @@ __global__ __launch_bounds__(256) void k_normalize_rows(NormArgs na, const doubl
 // A_hat = A_old + step (A - A_old) (Cpd.hpp:251-258).  The step depends on the current
 // acceleration coefficient, which the device updates: step_c for *coeff == c, else
 // step_c1 (the coefficient rises by at most one per sweep).
-__global__ void k_axpy_accel(int64_t n, double step_c, double step_c1, int c, const int *coeff, const double *old,
+// The same pass sets A_old = A for the next sweep (Cpd.hpp:348,353).
+__global__ void k_axpy_accel(int64_t n, double step_c, double step_c1, int c, const int *coeff, double *old,
                              const double *cur, double *out) {
   const double step = (coeff == nullptr || *coeff == c) ? step_c : step_c1;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
-    out[e] = old[e] + step * (cur[e] - old[e]);
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const double o = old[e], a = cur[e];
+    out[e] = o + step * (a - o);
+    old[e] = a;
+  }
 }
 
 // Line-search decision on the device (Cpd.hpp:262-269): accept the accelerated factors
@@ -512,7 +516,7 @@ void accel_decide(hipStream_t s, const double *scal, double thr, int *flags, con
   k_accel_decide<<<grid_for(cp.n[0] > 4096 ? cp.n[0] : 4096, 256), 256, 0, s>>>(scal, thr, flags, cp);
 }
 
-void axpy_accel(hipStream_t s, int64_t n, double step_c, double step_c1, int c, const int *coeff, const double *old,
+void axpy_accel(hipStream_t s, int64_t n, double step_c, double step_c1, int c, const int *coeff, double *old,
                 const double *cur, double *out) {
   k_axpy_accel<<<grid_for(n, 256), 256, 0, s>>>(n, step_c, step_c1, c, coeff, old, cur, out);
 }
