@@ -66,6 +66,7 @@ typedef struct {
   double   warm_kkt_tol;        /* a row takes the network's start if its relative KKT residual
                                    max|min(L a, a Zt - Wt)| / max|Wt| is <= this (default 1e-6) */
   int      mlp_harvest;         /* 1: append every NNLS (inputs, solution) pair to the handle's training set */
+  int      time_passes;         /* 1: HIP events around the two tensor passes of every sweep (t_pass_ms) */
 } ptn_options;
 
 /* Status<Tensor> (PARTENSOR_basic.hpp:556-595) minus the factors, which are written
@@ -90,6 +91,9 @@ typedef struct {
   double mu[PTN_MAX_ORDER];
   double t_end;        /* seconds from the start of the outer loop to the end of this sweep (device-synchronised) */
   int    warm_rows[PTN_MAX_ORDER];  /* rows whose NNLS started from the network's proposal */
+  double t_pass_ms[2]; /* time_passes: device time of the tensor passes launched in this sweep, [0] the
+                          left-node pass (X x_2 A_2, or its pair with the line search), [1] the right-node
+                          pass (dimension tree, FP32 contractions, order 3); 0 where none ran */
 } ptn_sweep_log;
 
 /* ---- handle ---- */

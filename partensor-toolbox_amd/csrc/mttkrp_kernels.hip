@@ -254,13 +254,21 @@ __global__ __launch_bounds__(256) void k_contract_rows(const TT *__restrict__ T,
 // rows at a time; the order of the sums is fixed.
 //   outmode 0: block (8 consecutive i0, i1-split), out[i0][r] = sum_i1 Y[m][r] F[i1][r]
 //   outmode 1: block (one i1, i0-split), 8 row groups summed in LDS in order
+// With gridDim.z == 2 the blocks of z = 1 contract (Y2, F2) beside (Y, F); partials are
+// part[split][z][i][ldr].
 // ---------------------------------------------------------------------------
 template <typename TT>
 __global__ __launch_bounds__(256) void k_contract2(const TT *__restrict__ Y, int64_t ld, int64_t D0, int64_t D1,
                                                    int outmode, const double *__restrict__ F, int ldr, int R,
                                                    int64_t slen, double *__restrict__ part, const int *sel,
-                                                   int64_t sel_off) {
+                                                   int64_t sel_off, const TT *__restrict__ Y2,
+                                                   const double *__restrict__ F2) {
   if (sel) Y += (int64_t)(*sel) * sel_off;              // node chosen on the device (line search)
+  if (blockIdx.z) {
+    Y = Y2;
+    F = F2;
+  }
+  const int64_t zs = (int64_t)blockIdx.y * gridDim.z + blockIdx.z;
   __shared__ double red[8][32];
   const int q = threadIdx.x >> 5, c0 = threadIdx.x & 31;
   const int NT = (R + 31) / 32;
@@ -283,7 +291,7 @@ __global__ __launch_bounds__(256) void k_contract2(const TT *__restrict__ Y, int
         acc = fma(y3, f3, acc);
       }
       for (; i1 < e; ++i1) acc = fma((double)Y[(i0 + D0 * i1) * ld + c], F[i1 * ldr + c], acc);
-      part[((int64_t)blockIdx.y * D0 + i0) * ldr + c] = acc;
+      part[(zs * D0 + i0) * ldr + c] = acc;
     }
     return;
   }
@@ -313,7 +321,7 @@ __global__ __launch_bounds__(256) void k_contract2(const TT *__restrict__ Y, int
     if (q == 0 && c < ldr) {
       double sum = red[0][c0];
       for (int g = 1; g < 8; ++g) sum += red[g][c0];
-      part[((int64_t)blockIdx.y * D1 + i1) * ldr + c] = sum;
+      part[(zs * D1 + i1) * ldr + c] = sum;
     }
   }
 }
@@ -384,8 +392,20 @@ void contract2(hipStream_t s, const double *Y64, const float *Y32, int64_t ld, i
   const int64_t D = outmode == 0 ? D1 : D0;             // contracted extent
   const int64_t slen = (D + splits - 1) / splits;
   dim3 grid(outmode == 0 ? (unsigned)((D0 + 7) / 8) : (unsigned)D1, (unsigned)splits);
-  if (Y64) k_contract2<double><<<grid, 256, 0, s>>>(Y64, ld, D0, D1, outmode, F, ldr, R, slen, part, sel, sel_off);
-  else     k_contract2<float><<<grid, 256, 0, s>>>(Y32, ld, D0, D1, outmode, F, ldr, R, slen, part, sel, sel_off);
+  if (Y64)
+    k_contract2<double><<<grid, 256, 0, s>>>(Y64, ld, D0, D1, outmode, F, ldr, R, slen, part, sel, sel_off, nullptr,
+                                             nullptr);
+  else
+    k_contract2<float><<<grid, 256, 0, s>>>(Y32, ld, D0, D1, outmode, F, ldr, R, slen, part, sel, sel_off, nullptr,
+                                            nullptr);
+}
+
+void contract2_pair(hipStream_t s, const float *Ya, const float *Yb, int64_t ld, int64_t D0, int64_t D1, int outmode,
+                    const double *Fa, const double *Fb, int ldr, int R, int splits, double *part) {
+  const int64_t D = outmode == 0 ? D1 : D0;
+  const int64_t slen = (D + splits - 1) / splits;
+  dim3 grid(outmode == 0 ? (unsigned)((D0 + 7) / 8) : (unsigned)D1, (unsigned)splits, 2);
+  k_contract2<float><<<grid, 256, 0, s>>>(Ya, ld, D0, D1, outmode, Fa, ldr, R, slen, part, nullptr, 0, Yb, Fb);
 }
 
 void reduce_splits(hipStream_t s, const double *partials, int splits, int64_t n, double *out) {

@@ -106,11 +106,13 @@ struct DevBuf {
 };
 
 template <typename T>
-struct PinnedBuf {                            // page-locked host memory (async device-to-host copies)
+struct PinnedBuf {      // page-locked, coherent host memory that kernels write directly (dev = its device address)
   T *p = nullptr;
+  T *dev = nullptr;
   explicit PinnedBuf(ptn_handle h) {
-    HIPCHK(hipHostMalloc((void **)&p, sizeof(T), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void **)&p, sizeof(T), hipHostMallocMapped | hipHostMallocCoherent));
     std::memset((void *)p, 0, sizeof(T));
+    HIPCHK(hipHostGetDevicePointer((void **)&dev, p, 0));
   }
   ~PinnedBuf() {
     if (p) (void)hipHostFree(p);
@@ -265,8 +267,8 @@ struct Work {
   int arith;
   bool f32;                                  // FP32 tensor stream (arith F32 / BF16X3 / F32_F64)
   bool mfma32;                               // contractions on FP32-output kernels (F32 / BF16X3)
-  DevBuf<double> fac, old, nrm, acc, grams, agr, Vm, Vacc, M, Aold, colsum, part, T, YL, nscr, scal, hist, betas,
-      gpart, cscr;
+  DevBuf<double> fac, old, acc, grams, agr, Vm, Vacc, M, M2, Aold, colsum, part, T, YL, nscr, scal, hist, betas,
+      gpart, gpart_side, cscr;
   DevBuf<float> T32, YL32, YL2, part32, wf;
   DevBuf<uint16_t> whi, wlo;
   DevBuf<NnlsParams> prm;
@@ -282,6 +284,17 @@ struct Work {
   } left, right;                             // dimension-tree nodes cached between the modes of a sweep
   bool left_ready = false;                   // left node already formed for the coming mode 0 (line-search pass)
   bool right_ready = false;                  // order-3 right node X x_0 A_0 already formed (aux stream)
+  bool m0_ready = false;                     // mode-0 MTTKRP already in M (paired pass, chosen on the device)
+  // time_passes: events around the sweep's tensor passes, slot = sweep parity ([2 q] start,
+  // [2 q + 1] end of pass q: 0 left node, 1 right node); tslot < 0 when not timing
+  hipEvent_t (*kev)[4] = nullptr;
+  bool kev_used[2][2] = {{false, false}, {false, false}};
+  int tslot = -1;
+  void pass_mark(int q, bool end, hipStream_t s) {
+    if (tslot < 0 || q < 0) return;
+    HIPCHK(hipEventRecord(kev[tslot][2 * q + (end ? 1 : 0)], s));
+    if (end) kev_used[tslot][q] = true;
+  }
 
   Work(ptn_handle hh, int R_, int arith_, bool need_tree)
       : h(hh), N(hh->N), R(R_), ldr(round_up16(R_)), rpf((R_ + 31) / 32 * 32), arith(arith_) {
@@ -297,13 +310,13 @@ struct Work {
       if (dims[k] > maxI) maxI = dims[k];
     }
     fsz = o;
-    fac.alloc(h, fsz); old.alloc(h, fsz); nrm.alloc(h, fsz); acc.alloc(h, fsz);
+    fac.alloc(h, fsz); old.alloc(h, fsz); acc.alloc(h, fsz);
     grams.alloc(h, (size_t)N * R * R); agr.alloc(h, (size_t)N * R * R);
     Vm.alloc(h, (size_t)(N + 1) * R * R); Vacc.alloc(h, (size_t)R * R);
     M.alloc(h, maxI * ldr); Aold.alloc(h, maxI * ldr); colsum.alloc(h, kMaxRank);
     prm.alloc(h, kMaxOrder + 1); mask.alloc(h, nnls_mask_words(maxI)); flags.alloc(h, 4);
     nscr.alloc(h, 2 * kMaxOrder * kMaxRank); scal.alloc(h, 8); betas.alloc(h, (size_t)(N + 1) * (kMaxInner + 1));
-    gpart.alloc(h, (size_t)((maxI + 31) / 32) * R * R);
+    gpart.alloc(h, (size_t)((maxI + 31) / 32) * R * R); gpart_side.alloc(h, (size_t)((maxI + 31) / 32) * R * R);
     cscr.alloc(h, (size_t)3 * R * R + R);    // R x R work of the unconstrained / orthogonality updates
     const int64_t C = total / dims[0];
     const int64_t yl = left_rows();
@@ -326,6 +339,7 @@ struct Work {
   void enable_pair() {
     const int64_t yl = left_rows();
     YL2.alloc(h, yl * 2 * rpf);
+    M2.alloc(h, 2 * dims[0] * ldr);          // [Mhat | M] of mode 0 from the paired pass
     if (N >= 4 && 64 * yl * 2 * rpf > part32_cap) {
       part32.~DevBuf(); new (&part32) DevBuf<float>();
       part32_cap = 64 * yl * 2 * rpf;
@@ -407,38 +421,48 @@ KrSpec kr_range(Work &w, double *base, int lo, int hi) {
 }
 
 // out[m][r] = sum_k X[m + M k] W(k)[r]: into dst (S == 1) or via split partials.
-Node node_cm(Work &w, const KrSpec &k, int64_t Mrows, int64_t K, int S, double *dst64, float *dst32) {
+// timed >= 0: the contraction kernel is bracketed by the events of pass `timed`
+Node node_cm(Work &w, const KrSpec &k, int64_t Mrows, int64_t K, int S, double *dst64, float *dst32,
+             int timed = -1) {
   ptn_handle h = w.h;
   hipStream_t s = h->stream;
   if (!w.mfma32) {
+    w.pass_mark(timed, false, s);
     if (S == 1) gemm_cm_f64(s, w.x64(), w.x32(), Mrows, K, k, w.ldr, 1, dst64);
     else {
       REQUIRE(S * Mrows * w.ldr <= w.part_cap, PTN_ERR_INVALID, "partial buffer too small");
       gemm_cm_f64(s, w.x64(), w.x32(), Mrows, K, k, w.ldr, S, w.part);
-      reduce_splits(s, w.part, S, Mrows * w.ldr, dst64);
     }
+    w.pass_mark(timed, true, s);
+    if (S > 1) reduce_splits(s, w.part, S, Mrows * w.ldr, dst64);
     return Node{dst64, nullptr, w.ldr};
   }
   w.pack(k, K, 0);
+  w.pass_mark(timed, false, s);
   if (S == 1) gemm_cm_f32(s, w.gmode(), w.x32(), Mrows, K, w.rpf, 1, w.whi, w.wlo, w.wf, dst32);
   else {
     REQUIRE(S * Mrows * w.rpf <= w.part32_cap, PTN_ERR_INVALID, "partial buffer too small");
     gemm_cm_f32(s, w.gmode(), w.x32(), Mrows, K, w.rpf, S, w.whi, w.wlo, w.wf, w.part32);
-    reduce_splits_f32(s, w.part32, S, Mrows * w.rpf, dst32);
   }
+  w.pass_mark(timed, true, s);
+  if (S > 1) reduce_splits_f32(s, w.part32, S, Mrows * w.rpf, dst32);
   return Node{nullptr, dst32, w.rpf};
 }
 
 // T[c][r] = sum_{a < L} X[a + L c] W(a)[r]
-Node node_ck(Work &w, const KrSpec &k, int64_t L, int64_t C, hipStream_t s = nullptr) {
+Node node_ck(Work &w, const KrSpec &k, int64_t L, int64_t C, hipStream_t s = nullptr, int timed = -1) {
   ptn_handle h = w.h;
   if (!s) s = h->stream;
   if (!w.mfma32) {
+    w.pass_mark(timed, false, s);
     gemm_ck_f64(s, w.x64(), w.x32(), L, C, k, w.ldr, w.T);
+    w.pass_mark(timed, true, s);
     return Node{w.T.p, nullptr, w.ldr};
   }
   w.pack(k, L, 1, nullptr, s);
+  w.pass_mark(timed, false, s);
   gemm_ck_f32(s, w.gmode(), w.x32(), L, C, w.rpf, w.whi, w.wlo, w.wf, w.T32, 0, nullptr, w.ldr, nullptr);
+  w.pass_mark(timed, true, s);
   return Node{nullptr, w.T32.p, w.rpf};
 }
 
@@ -450,8 +474,10 @@ void right_node_fused(Work &w, double *base, double *M) {
   const int64_t wpf = (d[1] + 63) / 64;
   REQUIRE(d[2] * wpf * w.ldr <= w.part_cap, PTN_ERR_INVALID, "partial buffer too small");
   w.pack(kr_of(w, base, {0}), d[0], 1);
+  w.pass_mark(1, false, h->stream);
   gemm_ck_f32(h->stream, w.gmode(), w.x32(), d[0], d[1] * d[2], w.rpf, w.whi, w.wlo, w.wf, nullptr, d[1],
               w.Fb(base, 1), w.ldr, w.part);
+  w.pass_mark(1, true, h->stream);
   reduce_chunks(h->stream, w.part, d[2], wpf, w.ldr, M);
 }
 
@@ -466,11 +492,13 @@ void contract_node(Work &w, const Node &nd, const KrSpec &sp, int outmode, doubl
   if (sp.nm == 2) {
     // two digits: coalesced row-block kernel (one output digit, one contracted digit)
     const int64_t D0 = sp.dim[0], D1 = sp.dim[1];
-    const int S = outmode == 0 ? pick_splits((D0 + 7) / 8, D1 / 16, 64) : pick_splits(D1, D0 / 64, 64);
+    int S = outmode == 0 ? pick_splits((D0 + 7) / 8, D1 / 16, 64) : pick_splits(D1, D0 / 64, 64);
+    // one block per output row already fills the device: no split, M written directly
+    if (outmode == 1 && D1 >= 512) S = 1;
     REQUIRE(S * In * w.ldr <= w.part_cap, PTN_ERR_INVALID, "partial buffer too small");
-    contract2(s, nd.p64, nd.p32, nd.ld, D0, D1, outmode, sp.fac[1 - outmode], w.ldr, w.R, S, w.part, nd.sel,
-              nd.sel_off);
-    reduce_splits(s, w.part, S, In * w.ldr, M);
+    contract2(s, nd.p64, nd.p32, nd.ld, D0, D1, outmode, sp.fac[1 - outmode], w.ldr, w.R, S, S == 1 ? M : w.part,
+              nd.sel, nd.sel_off);
+    if (S > 1) reduce_splits(s, w.part, S, In * w.ldr, M);
     return;
   }
   REQUIRE(nd.sel == nullptr, PTN_ERR_INVALID, "device-selected node needs two digits");
@@ -525,9 +553,13 @@ void mttkrp_tree(Work &w, double *base, int mode, double *M) {
   auto node = [](const Work::NodeRef &r) { return Node{r.p64, r.p32, r.ld, r.sel, r.sel_off}; };
   if (N == 3) {
     if (mode == 0 && !w.left_ready)
-      keep(w.left, node_cm(w, kr_of(w, base, {2}), d[0] * d[1], d[2], 1, w.YL.p, w.YL32.p));
+      keep(w.left, node_cm(w, kr_of(w, base, {2}), d[0] * d[1], d[2], 1, w.YL.p, w.YL32.p, 0));
     w.left_ready = false;
     if (mode <= 1) {
+      if (mode == 0 && w.m0_ready) {         // chosen with the line-search decision
+        w.m0_ready = false;
+        return;
+      }
       contract_node(w, node(w.left), kr_of(w, base, {0, 1}), mode, M);
       return;
     }
@@ -542,7 +574,7 @@ void mttkrp_tree(Work &w, double *base, int mode, double *M) {
       right_node_fused(w, base, M);
       return;
     }
-    keep(w.right, node_ck(w, kr_of(w, base, {0}), d[0], d[1] * d[2]));
+    keep(w.right, node_ck(w, kr_of(w, base, {0}), d[0], d[1] * d[2], nullptr, 1));
     contract_node(w, node(w.right), kr_of(w, base, {1, 2}), 1, M);
     return;
   }
@@ -553,10 +585,14 @@ void mttkrp_tree(Work &w, double *base, int mode, double *M) {
   const int64_t Ml = w.left_rows(), Kl = w.total / Ml;
   if (mode == 0 && !w.left_ready) {
     const int S = pick_splits((Ml + 255) / 256, Kl / 256, 64);
-    keep(w.left, node_cm(w, kr_range(w, base, hf, N), Ml, Kl, S, w.YL.p, w.YL32.p));
+    keep(w.left, node_cm(w, kr_range(w, base, hf, N), Ml, Kl, S, w.YL.p, w.YL32.p, 0));
   }
   w.left_ready = false;
-  if (mode == hf) keep(w.right, node_ck(w, kr_range(w, base, 0, hf), Ml, Kl));
+  if (mode == 0 && w.m0_ready) {             // chosen with the line-search decision
+    w.m0_ready = false;
+    return;
+  }
+  if (mode == hf) keep(w.right, node_ck(w, kr_range(w, base, 0, hf), Ml, Kl, nullptr, 1));
   if (mode < hf) contract_node(w, node(w.left), kr_range(w, base, 0, hf), mode, M);
   else contract_node(w, node(w.right), kr_range(w, base, hf, N), mode - hf, M);
 }
@@ -570,7 +606,7 @@ void right_node_early(Work &w, double *base) {
   ptn_handle h = w.h;
   HIPCHK(hipEventRecord(h->ev[2], h->stream));
   HIPCHK(hipStreamWaitEvent(h->aux, h->ev[2], 0));
-  const Node nd = node_ck(w, kr_of(w, base, {0}), w.dims[0], w.dims[1] * w.dims[2], h->aux);
+  const Node nd = node_ck(w, kr_of(w, base, {0}), w.dims[0], w.dims[1] * w.dims[2], h->aux, 1);
   w.right.p64 = nd.p64; w.right.p32 = nd.p32; w.right.ld = nd.ld; w.right.sel = nullptr; w.right.sel_off = 0;
   HIPCHK(hipEventRecord(h->ev[3], h->aux));
   w.right_ready = true;
@@ -578,11 +614,13 @@ void right_node_early(Work &w, double *base) {
 
 // Line-search pass fused with the next sweep's left node: one pass over X forms
 // YL2[m] = [X x_rest A_rest | X x_rest Ahat_rest] (rest = modes 2.. of the current and of
-// the accelerated factors, 2 rpf columns per row), and Mhat (I_0 x ldr) =
-// sum_{i1} YL2hat[i0 + I0 i1] Ahat_1[i1] from the upper half, so <Mhat, Ahat_0> is the
-// fit term <X, [[Ahat]]> of accel_cost_function (CpdDimTree.hpp:174-181) summed in
-// another order.
-void left_node_pair(Work &w, double *cur, double *acc, double *Mhat) {
+// the accelerated factors, 2 rpf columns per row), then one launch contracts both halves
+// over i1: M2 = [Mhat | M], Mhat (I_0 x ldr) = sum_{i1} YL2hat[i0 + I0 i1] Ahat_1[i1], so
+// <Mhat, Ahat_0> is the fit term <X, [[Ahat]]> of accel_cost_function
+// (CpdDimTree.hpp:174-181) summed in another order, and M the mode-0 MTTKRP of the current
+// factors (the leaf of the left node, as mttkrp_tree forms it).  The decision copies one of
+// the two into w.M.
+void left_node_pair(Work &w, double *cur, double *acc) {
   ptn_handle h = w.h;
   hipStream_t s = h->stream;
   const bool n3 = w.N == 3;
@@ -593,14 +631,23 @@ void left_node_pair(Work &w, double *cur, double *acc, double *Mhat) {
   const int width = 2 * w.rpf;
   w.pack(k1, K, 0, &k2);
   const int S = n3 ? 1 : pick_splits((Mrows + 255) / 256, K / 256, 64);
+  w.pass_mark(0, false, s);
   if (S == 1) {
     gemm_cm_f32(s, w.gmode(), w.x32(), Mrows, K, width, 1, w.whi, w.wlo, w.wf, w.YL2);
+    w.pass_mark(0, true, s);
   } else {
     REQUIRE(S * Mrows * width <= w.part32_cap, PTN_ERR_INVALID, "partial buffer too small");
     gemm_cm_f32(s, w.gmode(), w.x32(), Mrows, K, width, S, w.whi, w.wlo, w.wf, w.part32);
+    w.pass_mark(0, true, s);
     reduce_splits_f32(s, w.part32, S, Mrows * width, w.YL2);
   }
-  contract_node(w, Node{nullptr, w.YL2.p + w.rpf, width}, kr_range(w, acc, 0, hf), 0, Mhat);
+  const KrSpec la = kr_range(w, acc, 0, hf), lc = kr_range(w, cur, 0, hf);
+  const int64_t D0 = la.dim[0], D1 = la.dim[1];
+  const int S2 = pick_splits((D0 + 7) / 8, D1 / 16, 64);   // contract_node's split of outmode 0
+  REQUIRE(2 * S2 * D0 * w.ldr <= w.part_cap, PTN_ERR_INVALID, "partial buffer too small");
+  contract2_pair(s, w.YL2.p + w.rpf, w.YL2.p, width, D0, D1, 0, la.fac[1], lc.fac[1], w.ldr, w.R, S2, w.part);
+  reduce_splits(s, w.part, S2, 2 * D0 * w.ldr, w.M2);
+  w.m0_ready = true;
 }
 
 // Hadamard order of V_n: ascending (PartialCwiseProd, right-nested) for the standard
@@ -731,8 +778,8 @@ void update_nonneg(Work &w, int mode, const ptn_options &o, const ModeIn &in) {
   // without a warm start the start point A_old stays in hist[0] for the fallback
   nnls_iterate(s, I, w.R, w.ldr, in.V, p, in.betas, w.M, A, start, o.nesterov_delta_1, o.nesterov_delta_2,
                o.max_inner, w.mask, w.hist, w.colsum);
-  nnls_select(s, I, w.R, w.ldr, w.hist, w.mask, o.max_inner, A, w.colsum, p);
-  zero_col_fallback(s, I, w.R, w.ldr, w.colsum, warm ? w.Aold.p : w.hist.p, A, p);
+  nnls_finish(s, I, w.R, w.ldr, w.hist, w.mask, o.max_inner, A, w.colsum, warm ? w.Aold.p : w.hist.p, w.G(mode),
+              w.gpart, p);
   if (harvest) {
     // (V, M row, previous row) -> the solution row, appended to the training set
     mlp_features(s, I, w.R, w.ldr, in.V, w.M, w.Aold, A, ms->dx.p + ms->n * kMlpIn, ms->dt.p + ms->n * kMlpR,
@@ -742,7 +789,8 @@ void update_nonneg(Work &w, int mode, const ptn_options &o, const ModeIn &in) {
   }
 }
 
-// update_factor, Cpd.hpp:189-228: the constraint of the mode picks the update
+// update_factor, Cpd.hpp:189-228: the constraint of the mode picks the update; G_n =
+// A_n^T A_n follows (Cpd.hpp:227)
 void update_factor(Work &w, int mode, const ptn_options &o, const ModeIn &in) {
   ptn_handle h = w.h;
   hipStream_t s = h->stream;
@@ -768,6 +816,7 @@ void update_factor(Work &w, int mode, const ptn_options &o, const ModeIn &in) {
       break;
   }
   params_no_nnls(s, in.p);
+  gram_of(w, A, I, w.G(mode));
 }
 
 }  // namespace
@@ -1046,7 +1095,21 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
       int flags[4];
     };
     PinnedBuf<Readback> rbuf0(h), rbuf1(h);
+    struct PassEvents {                       // time_passes: 2 sweep slots x (start, end) x 2 passes
+      hipEvent_t e[2][4] = {};
+      ~PassEvents() {
+        for (auto &r : e)
+          for (hipEvent_t x : r)
+            if (x) (void)hipEventDestroy(x);
+      }
+    } pev;
+    if (o.time_passes) {
+      for (auto &r : pev.e)
+        for (hipEvent_t &x : r) HIPCHK(hipEventCreate(&x));
+      w.kev = pev.e;
+    }
     Readback *rbs[2] = {rbuf0.p, rbuf1.p};
+    Readback *rbd[2] = {rbuf0.dev, rbuf1.dev};     // the same buffers as the device addresses them
     DevBuf<double> snap0(h, w.fsz), snap1(h, w.fsz);
     double *snaps[2] = {snap0.p, snap1.p};
     EventPair evs(h);
@@ -1059,7 +1122,8 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
     bool tried_at[2] = {false, false};
 
     auto enqueue = [&](int it) {
-      Readback *rb = rbs[it & 1];
+      Readback *rbdev = rbd[it & 1];
+      w.tslot = w.kev ? (it & 1) : -1;
       for (int n = 0; n < N; ++n) {
         if (!(n == 0 && prepped0)) {
           sel[n] = slot(w, n);
@@ -1070,7 +1134,6 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
         else mttkrp_standard(w, w.fac, n, w.M);
         join();
         update_factor(w, n, o, sel[n]);
-        gram_of(w, w.F(n), w.dims[n], w.G(n));
         if (n == 0 && tree && w.mfma32 && N == 3) right_node_early(w, w.fac);
         if (n == last)
           cost_gram(ms, w.dims[last] * w.ldr, w.M, w.F(last), R, sel[last].V, w.G(last), w.scal.p, w.scal.p + 1, 0);
@@ -1081,12 +1144,18 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
         for (int k = 0; k < N; ++k) { gp[k] = w.G(k); fp[k] = w.F(k); }
         normalize(ms, N, w.dims, R, w.ldr, wf, gp, fp, w.nscr);
       }
-      HIPCHK(hipMemcpyAsync(rb->hp, w.prm.p, sizeof(NnlsParams) * (N + 1), hipMemcpyDeviceToHost, ms));
-      HIPCHK(hipMemcpyAsync(&rb->f, w.scal.p + 1, sizeof(double), hipMemcpyDeviceToHost, ms));
+      {
+        ExportList ex{};
+        ex.src[0] = (const unsigned *)w.prm.p; ex.dst[0] = (unsigned *)rbdev->hp;
+        ex.words[0] = (int)(sizeof(NnlsParams) * (N + 1) / 4);
+        ex.src[1] = (const unsigned *)(w.scal.p + 1); ex.dst[1] = (unsigned *)&rbdev->f; ex.words[1] = 2;
+        ex.count = 2;
+        export_words(ms, ex);
+      }
       // termination (Cpd.hpp:339-344): the iteration cap is known here; the threshold
       // test is repeated on the device, which then takes no step
       const bool cap = it >= (int)o.max_iter;
-      bool tried = false;
+      bool tried = false, snapped = false;
       if (!cap && o.acceleration) {
         if (it > 1) {
           // line_search_accel, Cpd.hpp:247-278: step (k+1)^(1/accel_coeff); the device's
@@ -1094,29 +1163,35 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
           tried = true;
           const double step_c = std::pow((double)(it + 1), 1.0 / (double)c_known);
           const double step_c1 = std::pow((double)(it + 1), 1.0 / (double)(c_known + 1));
-          // A_hat = A_old + step (A - A_old); A_old = A (the post-Normalize factors) in the same pass
-          axpy_accel(ms, w.fsz, step_c, step_c1, c_known, w.flags.p + 3, w.old, w.fac, w.acc);
-          for (int k = 0; k < N; ++k) gram_of(w, w.Fb(w.acc, k), w.dims[k], w.agr.p + (int64_t)k * R * R);
+          // A_hat = A_old + step (A - A_old); A_old = A (the post-Normalize factors) and the
+          // sweep's snapshot in the same pass
+          axpy_accel(ms, w.fsz, step_c, step_c1, c_known, w.flags.p + 3, w.old, w.fac, w.acc, snaps[it & 1]);
+          snapped = true;
+          // on the side stream: the accelerated factors' Grams and, for the paired pass,
+          // mode 0 of the next sweep for either outcome
+          fork();
+          if (pair) prep_mode(w, w.grams, 0, tree, slot(w, 0), o.max_inner, ss);
+          for (int k = 0; k < N; ++k)
+            gram(ss, w.dims[k], R, w.ldr, w.Fb(w.acc, k), w.agr.p + (int64_t)k * R * R, w.gpart_side);
+          if (pair) prep_mode(w, w.agr, 0, tree, slot(w, N), o.max_inner, ss);
           // accel_cost_function (CpdDimTree.hpp:174-181): its fit term from the last mode's
           // MTTKRP, or from mode 0 of the paired left-node pass
           const int cm = pair ? 0 : last;
+          const double *mfit = w.M;
           if (pair) {
-            // mode 0 of the next sweep for either outcome, beside the paired pass
-            fork();
-            prep_mode(w, w.grams, 0, tree, slot(w, 0), o.max_inner, ss);
-            prep_mode(w, w.agr, 0, tree, slot(w, N), o.max_inner, ss);
-            left_node_pair(w, w.fac, w.acc, w.M);
+            left_node_pair(w, w.fac, w.acc);
+            mfit = w.M2;
           } else {
             mttkrp_standard(w, w.acc, last, w.M);
           }
+          join();                    // the side stream has finished with the Grams and slots
           int list[kMaxOrder];
           int nl = v_order(N, cm, false, list);
           const double *gp[kMaxOrder];
           for (int j = 0; j < nl; ++j) gp[j] = w.agr.p + (int64_t)list[j] * R * R;
           hadamard(ms, R, nl, gp, w.Vacc);
-          cost_gram(ms, w.dims[cm] * w.ldr, w.M, w.Fb(w.acc, cm), R, w.Vacc, w.agr.p + (int64_t)cm * R * R,
+          cost_gram(ms, w.dims[cm] * w.ldr, mfit, w.Fb(w.acc, cm), R, w.Vacc, w.agr.p + (int64_t)cm * R * R,
                     w.scal.p, w.scal.p + 2, 1);
-          if (pair) join();          // the side stream has finished with the Grams and slots
           AccelCopy cp{};
           cp.src[0] = w.acc; cp.dst[0] = w.fac; cp.n[0] = w.fsz;
           cp.src[1] = w.agr; cp.dst[1] = w.grams; cp.n[1] = (int64_t)N * R * R;
@@ -1125,10 +1200,12 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
             cp.src[2] = a.V; cp.dst[2] = b.V; cp.n[2] = (int64_t)R * R;
             cp.src[3] = a.betas; cp.dst[3] = b.betas; cp.n[3] = kMaxInner + 1;
             cp.prm_src = a.p; cp.prm_dst = b.p;
+            const int64_t m0 = w.dims[0] * w.ldr;     // the mode-0 MTTKRP of the kept factors
+            cp.pick_src[0] = w.M2.p; cp.pick_src[1] = w.M2.p + m0; cp.pick_dst = w.M; cp.pick_n = m0;
           }
+          cp.fa_out = &rbdev->fa;
+          cp.flags_out = rbdev->flags;
           accel_decide(ms, w.scal.p, o.threshold_error, w.flags, cp);
-          HIPCHK(hipMemcpyAsync(&rb->fa, w.scal.p + 2, sizeof(double), hipMemcpyDeviceToHost, ms));
-          HIPCHK(hipMemcpyAsync(rb->flags, w.flags.p, 4 * sizeof(int), hipMemcpyDeviceToHost, ms));
           if (pair) {
             // the next sweep's left node: the half of the pair the device kept
             w.left.p64 = nullptr;
@@ -1145,7 +1222,7 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
           HIPCHK(hipMemcpyAsync(w.old, w.fac, w.fsz * sizeof(double), hipMemcpyDeviceToDevice, ms));
         }
       }
-      HIPCHK(hipMemcpyAsync(snaps[it & 1], w.fac, w.fsz * sizeof(double), hipMemcpyDeviceToDevice, ms));
+      if (!snapped) HIPCHK(hipMemcpyAsync(snaps[it & 1], w.fac, w.fsz * sizeof(double), hipMemcpyDeviceToDevice, ms));
       HIPCHK(hipEventRecord(evs.e[it & 1], ms));
       tried_at[it & 1] = tried;
     };
@@ -1166,6 +1243,13 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
         lg.mu[n] = rb->hp[n].mu0;
         lg.warm_rows[n] = rb->hp[n].warm_rows;
       }
+      for (int q = 0; q < 2 && w.kev; ++q)
+        if (w.kev_used[it & 1][q]) {              // the sweep's events are complete (its end event is)
+          float ms = 0.f;
+          HIPCHK(hipEventElapsedTime(&ms, w.kev[it & 1][2 * q], w.kev[it & 1][2 * q + 1]));
+          lg.t_pass_ms[q] = ms;
+          w.kev_used[it & 1][q] = false;
+        }
       const bool stop = r.rel_cost < o.threshold_error || it >= (int)o.max_iter;
       if (tried_at[it & 1] && !stop) {
         lg.accel_tried = 1;

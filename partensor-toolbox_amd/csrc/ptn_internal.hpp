@@ -162,6 +162,10 @@ void reduce_splits(hipStream_t s, const double *partials, int splits, int64_t n,
 void contract2(hipStream_t s, const double *Y64, const float *Y32, int64_t ld, int64_t D0, int64_t D1, int outmode,
                const double *F, int ldr, int R, int splits, double *part, const int *sel = nullptr,
                int64_t sel_off = 0);
+// contract2 of two FP32 nodes of one shape in one launch: part[split][0 / 1][i][ldr] for
+// (Ya, Fa) / (Yb, Fb); reduce_splits over 2 x D_out x ldr then gives both results.
+void contract2_pair(hipStream_t s, const float *Ya, const float *Yb, int64_t ld, int64_t D0, int64_t D1, int outmode,
+                    const double *Fa, const double *Fb, int ldr, int R, int splits, double *part);
 
 // FP32-stream MFMA variants (bf16x3 split products or FP32 MFMA), FP32 outputs with
 // RPF = R rounded up to 32 columns.  The Khatri-Rao operand is packed once per
@@ -203,23 +207,42 @@ void nnls_iterate(hipStream_t s, int64_t I, int R, int ldr, const double *V, con
                   int max_inner, uint64_t *mask, double *hist, double *colsum);   // zeroes colsum for nnls_select
 void nnls_select(hipStream_t s, int64_t I, int R, int ldr, const double *hist, const uint64_t *mask, int max_inner,
                  double *Anew, double *colsum, NnlsParams *p);
-void zero_col_fallback(hipStream_t s, int64_t I, int R, int ldr, const double *colsum, const double *Aold,
-                       double *A, NnlsParams *p);
+// nnls_select over 32-row blocks that also leave partial Grams (partials: ceil(I/32) x R x R),
+// then the zero-column test of update_factor (Cpd.hpp:206-207; A = 0.9 A + 0.1 Aold when a
+// column abs-sum is zero) and G = A^T A in gram's association: the end of a
+// non-negative factor update in two launches.
+void nnls_finish(hipStream_t s, int64_t I, int R, int ldr, const double *hist, const uint64_t *mask, int max_inner,
+                 double *A, double *colsum, const double *Aold, double *G, double *partials, NnlsParams *p);
 void cost_gram(hipStream_t s, int64_t nma, const double *M, const double *A, int R, const double *V, const double *G,
                const double *frob, double *f, int accel);
 void normalize(hipStream_t s, int N, const int64_t *dims, int R, int ldr, int wf, double *const *grams,
                double *const *fac, double *scratch);
 // Conditional copies of an accepted line-search step: four double ranges and one
-// NnlsParams (any may be empty / null).
+// NnlsParams (any may be empty / null).  pick: one range copied from pick_src[0] when the
+// step is accepted, else from pick_src[1].  fa_out / flags_out (mapped host memory,
+// optional): the accelerated cost and the four flags after the decision.
 struct AccelCopy {
   const double *src[4];
   double *dst[4];
   int64_t n[4];
   const NnlsParams *prm_src;
   NnlsParams *prm_dst;
+  const double *pick_src[2];
+  double *pick_dst;
+  int64_t pick_n;
+  double *fa_out;
+  int *flags_out;
 };
 void accel_decide(hipStream_t s, const double *scal, double thr, int *flags, const AccelCopy &cp);
 void axpy_accel(hipStream_t s, int64_t n, double step_c, double step_c1, int c, const int *coeff, double *old,
-                const double *cur, double *out);
+                const double *cur, double *out, double *snap = nullptr);
+// up to four ranges of 4-byte words copied to mapped host memory by one kernel
+struct ExportList {
+  const unsigned *src[4];
+  unsigned *dst[4];
+  int words[4];
+  int count;
+};
+void export_words(hipStream_t s, const ExportList &x);
 
 }  // namespace ptn

@@ -35,16 +35,7 @@ __global__ void k_hadamard(int R, int ng, GramPtrs G, double *V) {
 // per thread, rows staged in LDS, missing rows padded with zeros, which add exact
 // zeros); k_gram_combine adds the chunk sums in order.  Same association as the CPU
 // restatement.
-__global__ __launch_bounds__(256) void k_gram_chunks(int64_t I, int R, int ldr, const double *__restrict__ A,
-                                                     double *__restrict__ partials) {
-  __shared__ double As[32][kMaxRank + 4];
-  const int tid = threadIdx.x;
-  const int64_t i0 = (int64_t)blockIdx.x * 32;
-  for (int x = tid; x < 32 * (kMaxRank + 4); x += 256) {
-    const int i = x / (kMaxRank + 4), c = x % (kMaxRank + 4);
-    As[i][c] = (i0 + i < I && c < R) ? A[(i0 + i) * ldr + c] : 0.0;
-  }
-  __syncthreads();
+__device__ __forceinline__ void gram_rows32(const double (*As)[kMaxRank + 4], int R, int tid, double *o) {
   const int nb = (R + 3) / 4;                 // 4-wide blocks per dimension
   if (tid >= nb * nb) return;
   const int a0 = 4 * (tid / nb), b0 = 4 * (tid % nb);
@@ -63,12 +54,24 @@ __global__ __launch_bounds__(256) void k_gram_chunks(int64_t I, int R, int ldr, 
 #pragma unroll
       for (int y = 0; y < 4; ++y) acc[x][y] = fma(va[x], vb[y], acc[x][y]);
   }
-  double *o = partials + (int64_t)blockIdx.x * R * R;
 #pragma unroll
   for (int x = 0; x < 4; ++x)
 #pragma unroll
     for (int y = 0; y < 4; ++y)
       if (a0 + x < R && b0 + y < R) o[(a0 + x) * R + b0 + y] = acc[x][y];
+}
+
+__global__ __launch_bounds__(256) void k_gram_chunks(int64_t I, int R, int ldr, const double *__restrict__ A,
+                                                     double *__restrict__ partials) {
+  __shared__ double As[32][kMaxRank + 4];
+  const int tid = threadIdx.x;
+  const int64_t i0 = (int64_t)blockIdx.x * 32;
+  for (int x = tid; x < 32 * (kMaxRank + 4); x += 256) {
+    const int i = x / (kMaxRank + 4), c = x % (kMaxRank + 4);
+    As[i][c] = (i0 + i < I && c < R) ? A[(i0 + i) * ldr + c] : 0.0;
+  }
+  __syncthreads();
+  gram_rows32(As, R, tid, partials + (int64_t)blockIdx.x * R * R);
 }
 
 __global__ __launch_bounds__(256) void k_gram_combine(int R, int nchunks, const double *__restrict__ partials,
@@ -268,23 +271,99 @@ __global__ __launch_bounds__(256) void k_nnls_select(int64_t I, int R, int ldr, 
   if (wave == 0 && lane < R) atomicAdd(&colsum[lane], ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane]);
 }
 
-// min column abs-sum == 0 -> A = 0.9 A + 0.1 A_old (Cpd.hpp:206-207)
-__global__ void k_zero_col_fallback(int64_t I, int R, int ldr, const double *colsum, const double *Aold, double *A,
-                                    NnlsParams *prm) {
-  __shared__ int hit;
-  if (threadIdx.x == 0) {
-    double mn = INFINITY;
-    for (int r = 0; r < R; ++r) mn = colsum[r] < mn ? colsum[r] : mn;
-    hit = mn == 0.0;
-    prm->fallback = hit;
+// the zero-column fallback's mix (Cpd.hpp:206-207)
+__device__ __forceinline__ double zero_col_mix(double a, double aold) { return 0.9 * a + 0.1 * aold; }
+
+// k_nnls_select over the rows [32 b, 32 b + 32) of block b, which also leaves the
+// block's partial Gram of the selected rows (k_gram_chunks' association) in
+// partials[b]; k_gram_finish applies the zero-column test and combines.
+__global__ __launch_bounds__(256) void k_nnls_finish(int64_t I, int R, int ldr, const double *__restrict__ hist,
+                                                     const unsigned long long *__restrict__ mask, int max_inner,
+                                                     double *__restrict__ Anew, double *colsum,
+                                                     double *__restrict__ partials, NnlsParams *prm) {
+  __shared__ double As[32][kMaxRank + 4];
+  __shared__ unsigned long long mred[256];
+  __shared__ int sk;
+  const int tid = threadIdx.x;
+  {
+    const int64_t nblk = (I + 3) / 4;
+    unsigned long long m = ~0ull;
+    for (int64_t b = tid >> 2; b < nblk; b += 64) m &= mask[b * 4 + (tid & 3)];
+    mred[tid] = m;
+    __syncthreads();
+    for (int o = 128; o >= 4; o >>= 1) {
+      if (tid < o) mred[tid] &= mred[tid + o];
+      __syncthreads();
+    }
+    if (tid == 0) sk = first_common_check(mred, max_inner);
+    __syncthreads();
+  }
+  const int kstar = sk;
+  if (blockIdx.x == 0 && tid == 0) prm->iters = kstar;
+  const double *src = hist + (int64_t)kstar * I * ldr;
+  const int64_t i0 = (int64_t)blockIdx.x * 32;
+  for (int x = tid; x < 32 * (kMaxRank + 4); x += 256) {
+    const int i = x / (kMaxRank + 4), c = x % (kMaxRank + 4);
+    double v = 0.0;
+    if (i0 + i < I && c < ldr) {
+      v = src[(i0 + i) * ldr + c];
+      Anew[(i0 + i) * ldr + c] = v;
+    }
+    As[i][c] = c < R ? v : 0.0;
   }
   __syncthreads();
-  if (!hit) return;
+  if (tid < R) {
+    double cs = 0.0;
+    for (int i = 0; i < 32; ++i) cs += fabs(As[i][tid]);
+    atomicAdd(&colsum[tid], cs);
+  }
+  gram_rows32(As, R, tid, partials + (int64_t)blockIdx.x * R * R);
+}
+
+// G = A^T A from the chunk partials (k_gram_combine's order), after the zero-column test
+// of update_factor (Cpd.hpp:206-207): when some column abs-sum is zero, block 0 alone
+// recomputes G from the mixed rows 0.9 A + 0.1 A_old in the same association and then
+// stores the mixed rows.  prm->fallback records the test.
+__global__ __launch_bounds__(256) void k_gram_finish(int64_t I, int R, int ldr, int nchunks,
+                                                     const double *__restrict__ partials, double *__restrict__ G,
+                                                     const double *colsum, const double *Aold, double *A,
+                                                     NnlsParams *prm) {
+  double mn = INFINITY;
+  for (int r = 0; r < R; ++r) mn = colsum[r] < mn ? colsum[r] : mn;
+  const bool hit = mn == 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) prm->fallback = hit;
+  if (!hit) {
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < R * R; e += gridDim.x * blockDim.x) {
+      double g = partials[e];
+      int c = 1;
+      for (; c + 4 <= nchunks; c += 4) {
+        const double p0 = partials[(int64_t)c * R * R + e], p1 = partials[(int64_t)(c + 1) * R * R + e];
+        const double p2 = partials[(int64_t)(c + 2) * R * R + e], p3 = partials[(int64_t)(c + 3) * R * R + e];
+        g = (((g + p0) + p1) + p2) + p3;
+      }
+      for (; c < nchunks; ++c) g = g + partials[(int64_t)c * R * R + e];
+      G[e] = g;
+    }
+    return;
+  }
+  if (blockIdx.x != 0) return;
+  for (int e = threadIdx.x; e < R * R; e += blockDim.x) {
+    const int a = e / R, b = e % R;
+    double g = 0.0;
+    for (int c = 0; c < nchunks; ++c) {
+      double p = 0.0;
+      for (int64_t i = 32 * (int64_t)c; i < I && i < 32 * (int64_t)(c + 1); ++i)
+        p = fma(zero_col_mix(A[i * ldr + a], Aold[i * ldr + a]), zero_col_mix(A[i * ldr + b], Aold[i * ldr + b]), p);
+      g = c == 0 ? p : g + p;
+    }
+    G[e] = g;
+  }
+  __syncthreads();                            // every entry of G read A before it changes
   const int c = threadIdx.x & 63;
   if (c >= R) return;
   for (int64_t row = threadIdx.x >> 6; row < I; row += blockDim.x >> 6) {
     const int64_t e = row * ldr + c;
-    A[e] = 0.9 * A[e] + 0.1 * Aold[e];
+    A[e] = zero_col_mix(A[e], Aold[e]);
   }
 }
 
@@ -388,15 +467,23 @@ __global__ __launch_bounds__(256) void k_normalize_rows(NormArgs na, const doubl
 // A_hat = A_old + step (A - A_old) (Cpd.hpp:251-258).  The step depends on the current
 // acceleration coefficient, which the device updates: step_c for *coeff == c, else
 // step_c1 (the coefficient rises by at most one per sweep).
-// The same pass sets A_old = A for the next sweep (Cpd.hpp:348,353).
+// The same pass sets A_old = A for the next sweep (Cpd.hpp:348,353) and, when snap is
+// given, copies A there too.
 __global__ void k_axpy_accel(int64_t n, double step_c, double step_c1, int c, const int *coeff, double *old,
-                             const double *cur, double *out) {
+                             const double *cur, double *out, double *snap) {
   const double step = (coeff == nullptr || *coeff == c) ? step_c : step_c1;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
     const double o = old[e], a = cur[e];
     out[e] = o + step * (a - o);
     old[e] = a;
+    if (snap) snap[e] = a;
   }
+}
+
+// Small device-to-host copies in one launch (4-byte words; dst is mapped host memory).
+__global__ void k_export(ExportList x) {
+  for (int q = 0; q < x.count; ++q)
+    for (int e = threadIdx.x; e < x.words[q]; e += blockDim.x) x.dst[q][e] = x.src[q][e];
 }
 
 // Line-search decision on the device (Cpd.hpp:262-269): accept the accelerated factors
@@ -420,9 +507,18 @@ __global__ void k_accel_decide(const double *__restrict__ scal, double thr, int 
       }
       flags[2] = fail;
     }
+    if (cp.fa_out) {
+      *cp.fa_out = fa;
+      for (int q = 0; q < 4; ++q) cp.flags_out[q] = flags[q];
+    }
+  }
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  if (cp.pick_n) {
+    const double *src = accept ? cp.pick_src[0] : cp.pick_src[1];
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < cp.pick_n; e += stride)
+      cp.pick_dst[e] = src[e];
   }
   if (!accept) return;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int q = 0; q < 4; ++q)
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < cp.n[q]; e += stride)
       cp.dst[q][e] = cp.src[q][e];
@@ -483,17 +579,20 @@ void nnls_iterate(hipStream_t s, int64_t I, int R, int ldr, const double *V, con
 #undef PTN_NI
 }
 
+void nnls_finish(hipStream_t s, int64_t I, int R, int ldr, const double *hist, const uint64_t *mask, int max_inner,
+                 double *A, double *colsum, const double *Aold, double *G, double *partials, NnlsParams *p) {
+  const int nchunks = (int)((I + 31) / 32);
+  k_nnls_finish<<<(unsigned)nchunks, 256, 0, s>>>(I, R, ldr, hist, (const unsigned long long *)mask, max_inner, A,
+                                                  colsum, partials, p);
+  k_gram_finish<<<(unsigned)((R * R + 255) / 256), 256, 0, s>>>(I, R, ldr, nchunks, partials, G, colsum, Aold, A, p);
+}
+
 void nnls_select(hipStream_t s, int64_t I, int R, int ldr, const double *hist, const uint64_t *mask, int max_inner,
                  double *Anew, double *colsum, NnlsParams *p) {
   int64_t g = (I + 3) / 4;
   if (g > 256) g = 256;
   k_nnls_select<<<(unsigned)g, 256, 0, s>>>(I, R, ldr, hist, (const unsigned long long *)mask, max_inner, Anew,
                                             colsum, p);
-}
-
-void zero_col_fallback(hipStream_t s, int64_t I, int R, int ldr, const double *colsum, const double *Aold, double *A,
-                       NnlsParams *p) {
-  k_zero_col_fallback<<<1, 256, 0, s>>>(I, R, ldr, colsum, Aold, A, p);
 }
 
 void cost_gram(hipStream_t s, int64_t nma, const double *M, const double *A, int R, const double *V, const double *G,
@@ -517,8 +616,10 @@ void accel_decide(hipStream_t s, const double *scal, double thr, int *flags, con
 }
 
 void axpy_accel(hipStream_t s, int64_t n, double step_c, double step_c1, int c, const int *coeff, double *old,
-                const double *cur, double *out) {
-  k_axpy_accel<<<grid_for(n, 256), 256, 0, s>>>(n, step_c, step_c1, c, coeff, old, cur, out);
+                const double *cur, double *out, double *snap) {
+  k_axpy_accel<<<grid_for(n, 256), 256, 0, s>>>(n, step_c, step_c1, c, coeff, old, cur, out, snap);
 }
+
+void export_words(hipStream_t s, const ExportList &x) { k_export<<<1, 256, 0, s>>>(x); }
 
 }  // namespace ptn

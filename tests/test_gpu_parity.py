@@ -403,18 +403,24 @@ def test_dimtree_higher_orders(h, dims, R):
         _compare_runs(Fg, stg, logg[:tie], Fo, sto, logo[:tie], 1e-9, final=False)
 
 
-def test_zero_column_fallback_on_gpu(h):
-    say("zero-column fallback")
+@pytest.mark.parametrize("dims", [[6, 7, 8], [70, 37, 45]])
+def test_zero_column_fallback_on_gpu(h, dims):
+    """A negative tensor drives every Nesterov solve to zero: each mode takes the
+    0.9 A + 0.1 A_old fallback, and its Gram (one or several 32-row chunks) feeds the
+    next modes, so every factor and the cost check the fallback's Gram too."""
+    say("zero-column fallback", dims)
     rng = np.random.default_rng(12)
-    dims = [6, 7, 8]
     X = -o.cp_full([rng.random((I, 2)) for I in dims])
     init = [rng.random((I, 3)) for I in dims]
     h.set_tensor(X)
-    kw = dict(max_iter=1, threshold_error=0.0, normalization=0, acceleration=0)
+    kw = dict(max_iter=2, threshold_error=0.0, normalization=0, acceleration=0)
     Fg, stg, logg = h.cpd(3, init, **kw)
     Fo, sto, logo = o.cpd(X, 3, init, cost_form=o.COST_GRAM, **kw)
     assert logg[0]["zero_col_fallback"][0] == 1
-    assert np.array_equal(Fg[0], Fo[0])
+    assert [l["zero_col_fallback"] for l in logg] == [l["zero_col_fallback"] for l in logo]
+    for a, b in zip(Fg, Fo):
+        assert np.array_equal(a, b)
+    assert stg["f_value"] == pytest.approx(sto["f_value"], rel=1e-12)
 
 
 def test_invalid_arguments_raise(h):
