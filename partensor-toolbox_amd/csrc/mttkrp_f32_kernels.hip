@@ -344,7 +344,10 @@ __global__ __launch_bounds__(256) void k_gemm_ck_lds(const float *__restrict__ X
                                                      const double *__restrict__ A1, int ldr,
                                                      double *__restrict__ part) {
   constexpr int RPF = 32 * NT, TF = 256, LDP = 36;
-  __shared__ float tile[TF * LDP];
+  // unfused bf16x3: LDS padded to at most two blocks per CU (55 KB each), which measured
+  // faster beside the solver kernels of the other streams; the fused epilogue is not
+  constexpr int PAD = (MODE == kGemmBf16x3 && !FUSE) ? 4608 : 0;
+  __shared__ float tile[TF * LDP + PAD];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int i = lane & 31, kg = lane >> 5;
   int64_t cb, nfb, i1b = 0, i2 = 0;
