@@ -1132,50 +1132,51 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
         }
         if (tree) mttkrp_tree(w, w.fac, n, w.M);
         else mttkrp_standard(w, w.fac, n, w.M);
-        join();
+        if (!(n == 0 && prepped0)) join();   // else joined after the line-search pass
         update_factor(w, n, o, sel[n]);
         if (n == 0 && tree && w.mfma32 && N == 3) right_node_early(w, w.fac);
         if (n == last)
           cost_gram(ms, w.dims[last] * w.ldr, w.M, w.F(last), R, sel[last].V, w.G(last), w.scal.p, w.scal.p + 1, 0);
       }
       prepped0 = false;
-      if (o.normalization && !all_orth) {
-        double *gp[kMaxOrder], *fp[kMaxOrder];
-        for (int k = 0; k < N; ++k) { gp[k] = w.G(k); fp[k] = w.F(k); }
-        normalize(ms, N, w.dims, R, w.ldr, wf, gp, fp, w.nscr);
-      }
-      {
-        ExportList ex{};
-        ex.src[0] = (const unsigned *)w.prm.p; ex.dst[0] = (unsigned *)rbdev->hp;
-        ex.words[0] = (int)(sizeof(NnlsParams) * (N + 1) / 4);
-        ex.src[1] = (const unsigned *)(w.scal.p + 1); ex.dst[1] = (unsigned *)&rbdev->f; ex.words[1] = 2;
-        ex.count = 2;
-        export_words(ms, ex);
-      }
       // termination (Cpd.hpp:339-344): the iteration cap is known here; the threshold
       // test is repeated on the device, which then takes no step
       const bool cap = it >= (int)o.max_iter;
+      const bool step_now = !cap && o.acceleration && it > 1;
+      ExportList ex{};                 // the sweep's readback: NNLS parameters and the cost
+      ex.src[0] = (const unsigned *)w.prm.p; ex.dst[0] = (unsigned *)rbdev->hp;
+      ex.words[0] = (int)(sizeof(NnlsParams) * (N + 1) / 4);
+      ex.src[1] = (const unsigned *)(w.scal.p + 1); ex.dst[1] = (unsigned *)&rbdev->f; ex.words[1] = 2;
+      ex.count = 2;
+      // line_search_accel, Cpd.hpp:247-278: step (k+1)^(1/accel_coeff); the device's
+      // coefficient is c_known or, after one more fifth failure, c_known + 1.  A_hat =
+      // A_old + step (A - A_old); A_old = A (the post-Normalize factors) and the sweep's
+      // snapshot in the same pass (Normalize's row pass when it runs)
+      AxpyArgs ax{};
+      if (step_now) {
+        ax.step_c = std::pow((double)(it + 1), 1.0 / (double)c_known);
+        ax.step_c1 = std::pow((double)(it + 1), 1.0 / (double)(c_known + 1));
+        ax.c = c_known;
+        ax.coeff = w.flags.p + 3;
+        ax.cur = w.fac; ax.old = w.old; ax.out = w.acc; ax.snap = snaps[it & 1];
+      }
+      if (o.normalization && !all_orth) {
+        double *gp[kMaxOrder], *fp[kMaxOrder];
+        for (int k = 0; k < N; ++k) { gp[k] = w.G(k); fp[k] = w.F(k); }
+        normalize(ms, N, w.dims, R, w.ldr, wf, gp, fp, w.nscr, &ex, step_now ? &ax : nullptr);
+      } else {
+        export_words(ms, ex);
+        if (step_now) axpy_accel(ms, w.fsz, ax.step_c, ax.step_c1, ax.c, ax.coeff, w.old, w.fac, w.acc, ax.snap);
+      }
       bool tried = false, snapped = false;
       if (!cap && o.acceleration) {
         if (it > 1) {
-          // line_search_accel, Cpd.hpp:247-278: step (k+1)^(1/accel_coeff); the device's
-          // coefficient is c_known or, after one more fifth failure, c_known + 1
           tried = true;
-          const double step_c = std::pow((double)(it + 1), 1.0 / (double)c_known);
-          const double step_c1 = std::pow((double)(it + 1), 1.0 / (double)(c_known + 1));
-          // A_hat = A_old + step (A - A_old); A_old = A (the post-Normalize factors) and the
-          // sweep's snapshot in the same pass
-          axpy_accel(ms, w.fsz, step_c, step_c1, c_known, w.flags.p + 3, w.old, w.fac, w.acc, snaps[it & 1]);
           snapped = true;
-          // on the side stream: the accelerated factors' Grams and, for the paired pass,
-          // mode 0 of the next sweep for either outcome
+          // main stream first (the paired pass needs only A_hat), then the side stream:
+          // the accelerated factors' Grams and, for the paired pass, mode 0 of the next
+          // sweep for either outcome
           fork();
-          if (pair) prep_mode(w, w.grams, 0, tree, slot(w, 0), o.max_inner, ss);
-          for (int k = 0; k < N; ++k)
-            gram(ss, w.dims[k], R, w.ldr, w.Fb(w.acc, k), w.agr.p + (int64_t)k * R * R, w.gpart_side);
-          if (pair) prep_mode(w, w.agr, 0, tree, slot(w, N), o.max_inner, ss);
-          // accel_cost_function (CpdDimTree.hpp:174-181): its fit term from the last mode's
-          // MTTKRP, or from mode 0 of the paired left-node pass
           const int cm = pair ? 0 : last;
           const double *mfit = w.M;
           if (pair) {
@@ -1184,6 +1185,12 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
           } else {
             mttkrp_standard(w, w.acc, last, w.M);
           }
+          if (pair) prep_mode(w, w.grams, 0, tree, slot(w, 0), o.max_inner, ss);
+          for (int k = 0; k < N; ++k)
+            gram(ss, w.dims[k], R, w.ldr, w.Fb(w.acc, k), w.agr.p + (int64_t)k * R * R, w.gpart_side);
+          if (pair) prep_mode(w, w.agr, 0, tree, slot(w, N), o.max_inner, ss);
+          // accel_cost_function (CpdDimTree.hpp:174-181): its fit term from the last mode's
+          // MTTKRP, or from mode 0 of the paired left-node pass
           join();                    // the side stream has finished with the Grams and slots
           int list[kMaxOrder];
           int nl = v_order(N, cm, false, list);

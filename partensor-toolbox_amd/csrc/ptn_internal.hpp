@@ -215,8 +215,28 @@ void nnls_finish(hipStream_t s, int64_t I, int R, int ldr, const double *hist, c
                  double *A, double *colsum, const double *Aold, double *G, double *partials, NnlsParams *p);
 void cost_gram(hipStream_t s, int64_t nma, const double *M, const double *A, int R, const double *V, const double *G,
                const double *frob, double *f, int accel);
+
+// up to four ranges of 4-byte words copied to mapped host memory by one kernel
+struct ExportList {
+  const unsigned *src[4];
+  unsigned *dst[4];
+  int words[4];
+  int count;
+};
+void export_words(hipStream_t s, const ExportList &x);
+// the line-search step fused into Normalize's row pass (k_axpy_accel's arithmetic); cur is
+// the packed factor base the rows belong to, old / out / snap are indexed like it
+struct AxpyArgs {
+  double step_c, step_c1;
+  int c;
+  const int *coeff;
+  const double *cur;
+  double *old, *out, *snap;
+};
+// ex (optional): the sweep's readback, copied by the first kernel; ax (optional): the
+// line-search step in the row pass
 void normalize(hipStream_t s, int N, const int64_t *dims, int R, int ldr, int wf, double *const *grams,
-               double *const *fac, double *scratch);
+               double *const *fac, double *scratch, const ExportList *ex = nullptr, const AxpyArgs *ax = nullptr);
 // Conditional copies of an accepted line-search step: four double ranges and one
 // NnlsParams (any may be empty / null).  pick: one range copied from pick_src[0] when the
 // step is accepted, else from pick_src[1].  fa_out / flags_out (mapped host memory,
@@ -236,13 +256,6 @@ struct AccelCopy {
 void accel_decide(hipStream_t s, const double *scal, double thr, int *flags, const AccelCopy &cp);
 void axpy_accel(hipStream_t s, int64_t n, double step_c, double step_c1, int c, const int *coeff, double *old,
                 const double *cur, double *out, double *snap = nullptr);
-// up to four ranges of 4-byte words copied to mapped host memory by one kernel
-struct ExportList {
-  const unsigned *src[4];
-  unsigned *dst[4];
-  int words[4];
-  int count;
-};
-void export_words(hipStream_t s, const ExportList &x);
+
 
 }  // namespace ptn

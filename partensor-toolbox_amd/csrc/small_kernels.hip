@@ -406,7 +406,10 @@ struct NormArgs {
 
 // Column norms, scales and the Gram rescaling of Normalize (one block).  Columns whose
 // norms vanish keep scale 1 on every factor, so the row pass leaves them bit-identical.
-__global__ __launch_bounds__(256) void k_normalize(NormArgs na, double *scratch) {
+__global__ __launch_bounds__(256) void k_normalize(NormArgs na, double *scratch, ExportList ex) {
+  // the sweep's readback first (it does not depend on Normalize)
+  for (int q = 0; q < ex.count; ++q)
+    for (int e = threadIdx.x; e < ex.words[q]; e += blockDim.x) ex.dst[q][e] = ex.src[q][e];
   const int N = na.N, R = na.R, last = N - 1;
   double *lam = scratch;                       // (N-1) x R
   double *scale = scratch + kMaxOrder * kMaxRank;  // N x R column scales (1 = untouched)
@@ -455,13 +458,31 @@ __global__ __launch_bounds__(256) void k_normalize(NormArgs na, double *scratch)
 }
 
 // Row pass of Normalize: A_k[row][c] *= scale[k][c]; block (row tile of 4, factor k).
-__global__ __launch_bounds__(256) void k_normalize_rows(NormArgs na, const double *__restrict__ scratch) {
+// With ax.out the line-search step of k_axpy_accel follows for every entry of the row
+// (padding included): A_hat = A_old + step (A - A_old), A_old = A, snap = A.
+__global__ __launch_bounds__(256) void k_normalize_rows(NormArgs na, const double *__restrict__ scratch,
+                                                        AxpyArgs ax) {
   const int k = blockIdx.y, c = threadIdx.x & 63;
-  if ((k == na.N - 1 && k != na.wf) || c >= na.R) return;
   const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (row >= na.dims[k]) return;
-  const double sc = scratch[kMaxOrder * kMaxRank + k * na.R + c];
-  na.A[k][row * na.ldr + c] = na.A[k][row * na.ldr + c] * sc;
+  const bool scaled = !(k == na.N - 1 && k != na.wf) && c < na.R;
+  double *a = na.A[k] + row * na.ldr + c;
+  if (!ax.out) {
+    if (scaled) *a = *a * scratch[kMaxOrder * kMaxRank + k * na.R + c];
+    return;
+  }
+  if (c >= na.ldr) return;
+  double v = *a;
+  if (scaled) {
+    v = v * scratch[kMaxOrder * kMaxRank + k * na.R + c];
+    *a = v;
+  }
+  const double step = (ax.coeff == nullptr || *ax.coeff == ax.c) ? ax.step_c : ax.step_c1;
+  const int64_t e = (na.A[k] - ax.cur) + row * na.ldr + c;
+  const double o = ax.old[e];
+  ax.out[e] = o + step * (v - o);
+  ax.old[e] = v;
+  if (ax.snap) ax.snap[e] = v;
 }
 
 // A_hat = A_old + step (A - A_old) (Cpd.hpp:251-258).  The step depends on the current
@@ -601,14 +622,16 @@ void cost_gram(hipStream_t s, int64_t nma, const double *M, const double *A, int
 }
 
 void normalize(hipStream_t s, int N, const int64_t *dims, int R, int ldr, int wf, double *const *grams,
-               double *const *fac, double *scratch) {
+               double *const *fac, double *scratch, const ExportList *ex, const AxpyArgs *ax) {
   NormArgs na{};
   na.N = N; na.R = R; na.ldr = ldr; na.wf = wf;
   for (int k = 0; k < N; ++k) { na.dims[k] = dims[k]; na.G[k] = grams[k]; na.A[k] = fac[k]; }
   int64_t maxI = 0;
   for (int k = 0; k < N; ++k) maxI = dims[k] > maxI ? dims[k] : maxI;
-  k_normalize<<<1, 256, 0, s>>>(na, scratch);
-  k_normalize_rows<<<dim3((unsigned)((maxI + 3) / 4), (unsigned)N), 256, 0, s>>>(na, scratch);
+  ExportList e0{};
+  AxpyArgs a0{};
+  k_normalize<<<1, 256, 0, s>>>(na, scratch, ex ? *ex : e0);
+  k_normalize_rows<<<dim3((unsigned)((maxI + 3) / 4), (unsigned)N), 256, 0, s>>>(na, scratch, ax ? *ax : a0);
 }
 
 void accel_decide(hipStream_t s, const double *scal, double thr, int *flags, const AccelCopy &cp) {
