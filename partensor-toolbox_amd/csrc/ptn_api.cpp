@@ -1124,6 +1124,15 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
     auto enqueue = [&](int it) {
       Readback *rbdev = rbd[it & 1];
       w.tslot = w.kev ? (it & 1) : -1;
+      // termination (Cpd.hpp:339-344): the iteration cap is known here; the threshold
+      // test is repeated on the device, which then takes no step
+      const bool cap = it >= (int)o.max_iter;
+      const bool step_now = !cap && o.acceleration && it > 1;
+      // with the paired pass the sweep's cost is needed first by the line-search
+      // decision, so it runs on the side stream beside Normalize and the next pass
+      // (Normalize leaves the last factor and its Gram as they are unless it is the
+      // weight factor)
+      const bool side_cost = step_now && pair && !(o.normalization && !all_orth && wf == last);
       for (int n = 0; n < N; ++n) {
         if (!(n == 0 && prepped0)) {
           sel[n] = slot(w, n);
@@ -1135,19 +1144,18 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
         if (!(n == 0 && prepped0)) join();   // else joined after the line-search pass
         update_factor(w, n, o, sel[n]);
         if (n == 0 && tree && w.mfma32 && N == 3) right_node_early(w, w.fac);
-        if (n == last)
-          cost_gram(ms, w.dims[last] * w.ldr, w.M, w.F(last), R, sel[last].V, w.G(last), w.scal.p, w.scal.p + 1, 0);
+        if (n == last) {
+          if (side_cost) fork();
+          cost_gram(side_cost ? ss : ms, w.dims[last] * w.ldr, w.M, w.F(last), R, sel[last].V, w.G(last), w.scal.p,
+                    w.scal.p + 1, 0);
+        }
       }
       prepped0 = false;
-      // termination (Cpd.hpp:339-344): the iteration cap is known here; the threshold
-      // test is repeated on the device, which then takes no step
-      const bool cap = it >= (int)o.max_iter;
-      const bool step_now = !cap && o.acceleration && it > 1;
       ExportList ex{};                 // the sweep's readback: NNLS parameters and the cost
       ex.src[0] = (const unsigned *)w.prm.p; ex.dst[0] = (unsigned *)rbdev->hp;
       ex.words[0] = (int)(sizeof(NnlsParams) * (N + 1) / 4);
       ex.src[1] = (const unsigned *)(w.scal.p + 1); ex.dst[1] = (unsigned *)&rbdev->f; ex.words[1] = 2;
-      ex.count = 2;
+      ex.count = side_cost ? 1 : 2;    // with side_cost the decision exports the cost
       // line_search_accel, Cpd.hpp:247-278: step (k+1)^(1/accel_coeff); the device's
       // coefficient is c_known or, after one more fifth failure, c_known + 1.  A_hat =
       // A_old + step (A - A_old); A_old = A (the post-Normalize factors) and the sweep's
@@ -1191,12 +1199,12 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
           if (pair) prep_mode(w, w.agr, 0, tree, slot(w, N), o.max_inner, ss);
           // accel_cost_function (CpdDimTree.hpp:174-181): its fit term from the last mode's
           // MTTKRP, or from mode 0 of the paired left-node pass
-          join();                    // the side stream has finished with the Grams and slots
           int list[kMaxOrder];
           int nl = v_order(N, cm, false, list);
           const double *gp[kMaxOrder];
           for (int j = 0; j < nl; ++j) gp[j] = w.agr.p + (int64_t)list[j] * R * R;
-          hadamard(ms, R, nl, gp, w.Vacc);
+          hadamard(ss, R, nl, gp, w.Vacc);
+          join();                    // the side stream has finished with the Grams, V_hat and slots
           cost_gram(ms, w.dims[cm] * w.ldr, mfit, w.Fb(w.acc, cm), R, w.Vacc, w.agr.p + (int64_t)cm * R * R,
                     w.scal.p, w.scal.p + 2, 1);
           AccelCopy cp{};
@@ -1212,6 +1220,7 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
           }
           cp.fa_out = &rbdev->fa;
           cp.flags_out = rbdev->flags;
+          cp.f_out = side_cost ? &rbdev->f : nullptr;
           accel_decide(ms, w.scal.p, o.threshold_error, w.flags, cp);
           if (pair) {
             // the next sweep's left node: the half of the pair the device kept

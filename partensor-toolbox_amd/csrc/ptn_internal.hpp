@@ -252,6 +252,7 @@ struct AccelCopy {
   int64_t pick_n;
   double *fa_out;
   int *flags_out;
+  double *f_out;                             // optional: the sweep's cost
 };
 void accel_decide(hipStream_t s, const double *scal, double thr, int *flags, const AccelCopy &cp);
 void axpy_accel(hipStream_t s, int64_t n, double step_c, double step_c1, int c, const int *coeff, double *old,

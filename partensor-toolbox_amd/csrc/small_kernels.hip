@@ -532,6 +532,7 @@ __global__ void k_accel_decide(const double *__restrict__ scal, double thr, int 
       *cp.fa_out = fa;
       for (int q = 0; q < 4; ++q) cp.flags_out[q] = flags[q];
     }
+    if (cp.f_out) *cp.f_out = f;
   }
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   if (cp.pick_n) {
