@@ -46,7 +46,7 @@ def timed_sweeps(status, sweep_log, W, K):
     return t_end[W + K - 1] - t0
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_left_node_800cubed_R32.md")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_tensor_passes_800cubed_R32.md")
 
 
 def measured_traffic(key="left_pair_traffic_bytes"):
