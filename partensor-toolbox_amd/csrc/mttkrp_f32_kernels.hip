@@ -111,18 +111,19 @@ __device__ __forceinline__ void mma_step(const float (&x)[8], const bf16x8 &bhi,
 // values of every lane (unconditional loads from clamped addresses), converts and
 // multiplies; occupancy hides the load latency.  The 64 x 32 result of each column tile
 // leaves through LDS as 16-byte non-temporal row stores: scattered 4-byte stores of the
-// MFMA layout cost several times their bytes next to the tensor stream.  In bf16x3 mode
-// the staging area is padded so that at most two blocks share a CU: the strided tensor
-// stream runs 3-7% faster with 8 waves per CU than with the 20 the registers allow
-// (measured at 800^3).
+// MFMA layout cost several times their bytes next to the tensor stream.  CAP (bf16x3,
+// launches without a k split): the staging area is padded so that at most two blocks
+// share a CU; the strided tensor stream of an 800^3 left node runs 3-10% faster with 8
+// waves per CU than with the 20 the registers allow, while the k-split order-4 node
+// (200^4) runs 4% slower, so split launches keep the full occupancy.
 // ---------------------------------------------------------------------------
-template <int MODE, int NT>
+template <int MODE, int NT, bool CAP>
 __global__ __launch_bounds__(256) void k_gemm_cm_f32(const float *__restrict__ X, int64_t M, int64_t K,
                                                      int64_t kchunk, const bf16x8 *__restrict__ whi,
                                                      const bf16x8 *__restrict__ wlo, const float *__restrict__ wf,
                                                      float *__restrict__ out) {
   constexpr int RPF = 32 * NT;
-  constexpr int PAD = MODE == kGemmBf16x3 ? 2048 : 0;     // floats per wave: 66.5 KB per block
+  constexpr int PAD = CAP ? 2048 : 0;                    // floats per wave: 66.5 KB per block
   __shared__ float stg[4][64 * 33 + PAD];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int i = lane & 31, kg = lane >> 5;
@@ -513,15 +514,17 @@ void gemm_cm_f32(hipStream_t s, GemmMode mode, const float *x32, int64_t M, int6
   kchunk = (kchunk + 15) / 16 * 16;
   dim3 grid((unsigned)((M + 255) / 256), (unsigned)splits);
   const bf16x8 *h = (const bf16x8 *)whi, *l = (const bf16x8 *)wlo;
-#define PTN_CM32(MODEV, NTV) k_gemm_cm_f32<MODEV, NTV><<<grid, 256, 0, s>>>(x32, M, K, kchunk, h, l, wf, out)
+#define PTN_CM32(MODEV, NTV, CAPV) \
+  k_gemm_cm_f32<MODEV, NTV, CAPV><<<grid, 256, 0, s>>>(x32, M, K, kchunk, h, l, wf, out)
   if (mode == kGemmBf16x3) {
-    if (RPF == 32) PTN_CM32(kGemmBf16x3, 1);
-    else if (RPF == 64) PTN_CM32(kGemmBf16x3, 2);
-    else PTN_CM32(kGemmBf16x3, 4);
+    const bool cap = splits == 1;
+    if (RPF == 32) { if (cap) PTN_CM32(kGemmBf16x3, 1, true); else PTN_CM32(kGemmBf16x3, 1, false); }
+    else if (RPF == 64) { if (cap) PTN_CM32(kGemmBf16x3, 2, true); else PTN_CM32(kGemmBf16x3, 2, false); }
+    else PTN_CM32(kGemmBf16x3, 4, false);
   } else {
-    if (RPF == 32) PTN_CM32(kGemmFp32, 1);
-    else if (RPF == 64) PTN_CM32(kGemmFp32, 2);
-    else PTN_CM32(kGemmFp32, 4);
+    if (RPF == 32) PTN_CM32(kGemmFp32, 1, false);
+    else if (RPF == 64) PTN_CM32(kGemmFp32, 2, false);
+    else PTN_CM32(kGemmFp32, 4, false);
   }
 #undef PTN_CM32
 }
