@@ -187,8 +187,10 @@ ptn_status ptn_batch_generate(ptn_handle h, int nprob, const int *dims, const in
 ptn_status ptn_batch_init_factors(ptn_handle h, uint64_t seed, double *packed_out);
 /* Tensor p of the batch as stored (FP32 values widened to FP64), column-major. */
 ptn_status ptn_batch_get_tensor(ptn_handle h, int p, double *X_out);
-/* ALS on every problem of the batch with the options of ptn_cpd (mttkrp_form, arith and
-   fuse_accel are not consulted).  init_packed / out_packed: factors packed per problem,
+/* ALS on every problem of the batch with the options of ptn_cpd (mttkrp_form and
+   fuse_accel are not consulted; arith: PTN_ARITH_F64 runs the MTTKRPs on the FP64 matrix
+   cores, PTN_ARITH_F32 on the FP32 matrix cores with FP64 totals every 32 products, all
+   else FP64 either way).  init_packed / out_packed: factors packed per problem,
    problems back to back; res: nprob records; log: NULL or [nprob][opt->max_iter].
    block / grid: launch shape of the sweep kernel (256 or 512 threads), or both 0 to let
    the autotuner time a few shapes on the first two sweeps (state restored afterwards).

@@ -105,7 +105,14 @@ __device__ __forceinline__ void gram_block(const double *A, int I, int R, double
 // (stream-K): a wave stores the bands its range covers whole straight into M and keeps the
 // (at most two) partial bands in registers; mttkrp_mfma_partials / mttkrp_mfma_reduce add
 // them up in wave order (a fixed order: the result does not depend on the grid).
+// F32: v_mfma_f32_16x16x4f32 on the FP32 tensor values and the FP32-rounded Khatri-Rao
+// values (twice the FP64 matrix-core rate); each group of kSteps steps accumulates in FP32
+// and is then added to the FP64 totals.  Its D layout is lane l, register j -> row
+// 4 (l / 16) + j (the FP64 instruction's is row l / 16 + 4 j), both checked on gfx950.
 typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+template <bool F32>
+__device__ __forceinline__ int dtile_row(int kk, int j) { return F32 ? 4 * kk + j : kk + 4 * j; }
 constexpr int kSteps = 8;                   // k-steps whose tensor loads a lane issues together
 
 struct MfmaPart {
@@ -113,7 +120,7 @@ struct MfmaPart {
   int band0, band1;
 };
 
-template <int MODE>
+template <int MODE, bool F32>
 __device__ __forceinline__ void mttkrp_mfma(const float *X, int I0, int I1, int I2, const double *Fa,
                                             const double *Fb, double *M, int nw, MfmaPart &part) {
   const int t = tid(), lane = t & 63, wave = t >> 6;
@@ -163,26 +170,57 @@ __device__ __forceinline__ void mttkrp_mfma(const float *X, int I0, int I1, int 
         ibs[u] = ib;
         advance();
       }
+      if constexpr (F32) {
+        f32x4 e0 = {0.f, 0.f, 0.f, 0.f}, e1 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int u = 0; u < kSteps; ++u) {
-        const double bv = Fa[ias[u] * kBR + m] * Fb[ibs[u] * kBR + m];
-        c0 = __builtin_amdgcn_mfma_f64_16x16x4f64((double)xa[u], bv, c0, 0, 0, 0);
-        c1 = __builtin_amdgcn_mfma_f64_16x16x4f64((double)xb[u], bv, c1, 0, 0, 0);
+        for (int u = 0; u < kSteps; ++u) {
+          const float bv = (float)(Fa[ias[u] * kBR + m] * Fb[ibs[u] * kBR + m]);
+          e0 = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[u], bv, e0, 0, 0, 0);
+          e1 = __builtin_amdgcn_mfma_f32_16x16x4f32(xb[u], bv, e1, 0, 0, 0);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          c0[j] = c0[j] + (double)e0[j];
+          c1[j] = c1[j] + (double)e1[j];
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < kSteps; ++u) {
+          const double bv = Fa[ias[u] * kBR + m] * Fb[ibs[u] * kBR + m];
+          c0 = __builtin_amdgcn_mfma_f64_16x16x4f64((double)xa[u], bv, c0, 0, 0, 0);
+          c1 = __builtin_amdgcn_mfma_f64_16x16x4f64((double)xb[u], bv, c1, 0, 0, 0);
+        }
       }
     }
-    for (int64_t j = ef - n; j < e; ++j) {
-      const bool v = k < K;
-      const int off = v ? offset(k, ia, ib) : 0;
-      const double bv = v ? Fa[ia * kBR + m] * Fb[ib * kBR + m] : 0.0;
-      const double av = v ? (double)x0[off] : 0.0, aw = v ? (double)x1[off] : 0.0;
-      c0 = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, c0, 0, 0, 0);
-      c1 = __builtin_amdgcn_mfma_f64_16x16x4f64(aw, bv, c1, 0, 0, 0);
-      advance();
+    {
+      f32x4 e0 = {0.f, 0.f, 0.f, 0.f}, e1 = {0.f, 0.f, 0.f, 0.f};   // F32: the tail's group
+      for (int64_t j = ef - n; j < e; ++j) {
+        const bool v = k < K;
+        const int off = v ? offset(k, ia, ib) : 0;
+        const double bv = v ? Fa[ia * kBR + m] * Fb[ib * kBR + m] : 0.0;
+        if constexpr (F32) {
+          const float av = v ? x0[off] : 0.f, aw = v ? x1[off] : 0.f;
+          e0 = __builtin_amdgcn_mfma_f32_16x16x4f32(av, (float)bv, e0, 0, 0, 0);
+          e1 = __builtin_amdgcn_mfma_f32_16x16x4f32(aw, (float)bv, e1, 0, 0, 0);
+        } else {
+          const double av = v ? (double)x0[off] : 0.0, aw = v ? (double)x1[off] : 0.0;
+          c0 = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, c0, 0, 0, 0);
+          c1 = __builtin_amdgcn_mfma_f64_16x16x4f64(aw, bv, c1, 0, 0, 0);
+        }
+        advance();
+      }
+      if constexpr (F32) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          c0[j] = c0[j] + (double)e0[j];
+          c1[j] = c1[j] + (double)e1[j];
+        }
+      }
     }
     if (s == tb && e == te) {
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const int r = band * 32 + kk + 4 * j;
+        const int r = band * 32 + dtile_row<F32>(kk, j);
         if (r < Iout) M[r * kBR + m] = c0[j];
         if (r + 16 < Iout) M[(r + 16) * kBR + m] = c1[j];
       }
@@ -201,6 +239,7 @@ __device__ __forceinline__ void mttkrp_mfma(const float *X, int I0, int I1, int 
 }
 
 // the partial bands of every wave into P ([wave][slot][32 x 16]) and their ids into seg
+template <bool F32>
 __device__ __forceinline__ void mttkrp_mfma_partials(const MfmaPart &part, int nw, double *P, int *seg) {
   const int t = tid(), lane = t & 63, wave = t >> 6;
   if (wave >= nw) return;
@@ -212,7 +251,7 @@ __device__ __forceinline__ void mttkrp_mfma_partials(const MfmaPart &part, int n
   double *p0 = P + (2 * wave) * 32 * kBR, *p1 = p0 + 32 * kBR;
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    const int r = kk + 4 * j;
+    const int r = dtile_row<F32>(kk, j);
     if (part.band0 >= 0) {
       p0[r * kBR + m] = part.a0[j];
       p0[(r + 16) * kBR + m] = part.a1[j];
@@ -244,16 +283,16 @@ __device__ __forceinline__ void mttkrp_mfma_reduce(int Iout, int nw, const doubl
 
 // one mode's MTTKRP into S.M with operands S.F[0], S.F[1]: waves [0, nw) compute, `other`
 // runs on the remaining waves meanwhile; P holds 2 x 32 x 16 doubles per wave
-template <int BS, int MODE, typename Other>
+template <int BS, int MODE, bool F32, typename Other>
 __device__ __forceinline__ void mttkrp_block(const float *X, int I0, int I1, int I2, BatchLds &S, double *P, int nw,
                                              Other &&other) {
   MfmaPart part;
-  if ((int)(tid() >> 6) < nw) mttkrp_mfma<MODE>(X, I0, I1, I2, S.F[0], S.F[1], S.M, nw, part);
+  if ((int)(tid() >> 6) < nw) mttkrp_mfma<MODE, F32>(X, I0, I1, I2, S.F[0], S.F[1], S.M, nw, part);
   else {
     part.band0 = part.band1 = -1;
     other();
   }
-  mttkrp_mfma_partials(part, nw, P, S.seg);
+  mttkrp_mfma_partials<F32>(part, nw, P, S.seg);
   __syncthreads();
   mttkrp_mfma_reduce<BS>(MODE == 0 ? I0 : MODE == 1 ? I1 : I2, nw, P, S.seg, S.M);
   __syncthreads();
@@ -371,7 +410,7 @@ __device__ __forceinline__ int nnls_block(int I, int R, BatchLds &S, double *Ag,
 }
 
 // one sweep of one problem
-template <int BS, int RP>
+template <int BS, int RP, bool F32>
 __device__ __forceinline__ void sweep_problem(const BatchArgs &a, int p, int sweep, BatchLds &S, double *P) {
   constexpr int NW = BS / 64;                  // the last wave solves the eigenproblem meanwhile
   const BatchDesc &d = a.desc[p];
@@ -402,9 +441,9 @@ __device__ __forceinline__ void sweep_problem(const BatchArgs &a, int p, int swe
       eig_extremes_wave<kBR>(R, S.eig, amax, amin);
       if (l == 0) glambda_betas(amax, amin, &S.prm, a.max_inner, S.betas);
     };
-    if (n == 0) mttkrp_block<BS, 0>(X, I0, I1, I2, S, P, NW - 1, eig);
-    else if (n == 1) mttkrp_block<BS, 1>(X, I0, I1, I2, S, P, NW - 1, eig);
-    else mttkrp_block<BS, 2>(X, I0, I1, I2, S, P, NW - 1, eig);
+    if (n == 0) mttkrp_block<BS, 0, F32>(X, I0, I1, I2, S, P, NW - 1, eig);
+    else if (n == 1) mttkrp_block<BS, 1, F32>(X, I0, I1, I2, S, P, NW - 1, eig);
+    else mttkrp_block<BS, 2, F32>(X, I0, I1, I2, S, P, NW - 1, eig);
     const int In = n == 0 ? I0 : n == 1 ? I1 : I2;
     const int it = nnls_block<BS, RP>(In, R, S, n == 0 ? A0 : n == 1 ? A1 : A2, a.d1, a.d2, a.max_inner);
     if (t == 0) S.iters[n] = it;
@@ -471,7 +510,7 @@ __device__ __forceinline__ void sweep_problem(const BatchArgs &a, int p, int swe
       __syncthreads();
       gram_block<BS>(S.F[0], I0, R, S.Gh[0]);
       gram_block<BS>(S.F[1], I1, R, S.Gh[1]);
-      mttkrp_block<BS, 2>(X, I0, I1, I2, S, P, NW, [] {});
+      mttkrp_block<BS, 2, F32>(X, I0, I1, I2, S, P, NW, [] {});
       double ma = 0.0, vg = 0.0;
       for (int e = t; e < I2 * kBR; e += BS) ma = fma(S.M[e], Ah2[e], ma);
       if (t < kBR * kBR) vg = (S.Gh[1][t] * S.Gh[0][t]) * S.Gh[2][t];
@@ -515,7 +554,7 @@ __device__ __forceinline__ void sweep_problem(const BatchArgs &a, int p, int swe
   __syncthreads();
 }
 
-template <int BS>
+template <int BS, bool F32>
 __global__ __launch_bounds__(BS, BS == 512 ? 2 : 1) void k_batch_sweep(BatchArgs a) {
   __shared__ BatchLds S;
   __shared__ double P[(BS / 64) * 2 * 32 * kBR];     // partial MTTKRP bands, per wave
@@ -529,10 +568,10 @@ __global__ __launch_bounds__(BS, BS == 512 ? 2 : 1) void k_batch_sweep(BatchArgs
     if (q >= a.nprob) break;
     const int p = a.order[q];
     switch ((a.desc[p].R + 3) >> 2) {
-      case 1: sweep_problem<BS, 4>(a, p, sweep, S, P); break;
-      case 2: sweep_problem<BS, 8>(a, p, sweep, S, P); break;
-      case 3: sweep_problem<BS, 12>(a, p, sweep, S, P); break;
-      default: sweep_problem<BS, 16>(a, p, sweep, S, P); break;
+      case 1: sweep_problem<BS, 4, F32>(a, p, sweep, S, P); break;
+      case 2: sweep_problem<BS, 8, F32>(a, p, sweep, S, P); break;
+      case 3: sweep_problem<BS, 12, F32>(a, p, sweep, S, P); break;
+      default: sweep_problem<BS, 16, F32>(a, p, sweep, S, P); break;
     }
   }
 }
@@ -573,8 +612,13 @@ size_t batch_lds_bytes() { return sizeof(BatchLds); }
 void batch_init(hipStream_t s, const BatchArgs &a) { k_batch_init<256><<<a.nprob, 256, 0, s>>>(a); }
 
 void batch_sweep(hipStream_t s, const BatchArgs &a, int block, int grid) {
-  if (block == 512) k_batch_sweep<512><<<grid, 512, 0, s>>>(a);
-  else k_batch_sweep<256><<<grid, 256, 0, s>>>(a);
+  if (a.arith_f32) {
+    if (block == 512) k_batch_sweep<512, true><<<grid, 512, 0, s>>>(a);
+    else k_batch_sweep<256, true><<<grid, 256, 0, s>>>(a);
+  } else {
+    if (block == 512) k_batch_sweep<512, false><<<grid, 512, 0, s>>>(a);
+    else k_batch_sweep<256, false><<<grid, 256, 0, s>>>(a);
+  }
 }
 
 void batch_next_sweep(hipStream_t s, int *ctr) { k_batch_next<<<1, 64, 0, s>>>(ctr); }

@@ -1700,6 +1700,8 @@ ptn_status ptn_cpd_batch(ptn_handle h, const double *init_packed, const ptn_opti
       REQUIRE(o.constraints[k] == PTN_NONNEGATIVITY, PTN_ERR_UNSUPPORTED, "only the nonnegativity constraint is built");
     REQUIRE((block == 0 && grid == 0) || ((block == 256 || block == 512) && grid >= 1 && grid <= (1 << 16)),
             PTN_ERR_INVALID, "block/grid: both 0 (autotune) or block 256 / 512 with grid in [1, 65536]");
+    REQUIRE(o.arith == PTN_ARITH_F64 || o.arith == PTN_ARITH_F32, PTN_ERR_UNSUPPORTED,
+            "batch MTTKRP arithmetic: PTN_ARITH_F64 (FP64 matrix cores) or PTN_ARITH_F32 (FP32 matrix cores)");
     const BatchSet &b = *h->batch;
     const int P = b.nprob;
     const int64_t nr = b.nrows * kBatchBR;
@@ -1771,6 +1773,7 @@ ptn_status ptn_cpd_batch(ptn_handle h, const double *init_packed, const ptn_opti
     a.pow_table = powt.p;
     a.pow_stride = stride;
     a.log = nullptr;
+    a.arith_f32 = o.arith == PTN_ARITH_F32 ? 1 : 0;
     batch_init(s, a);                                   // ||X_p||^2 and the initial Grams
 
     int cu = 0;

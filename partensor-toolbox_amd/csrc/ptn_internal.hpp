@@ -79,6 +79,7 @@ struct BatchArgs {
   const double *pow_table;     // (sweep + 1)^(1 / coeff) at [sweep * pow_stride + coeff]
   int pow_stride;
   BatchLog *log;               // [nprob][max_iter] (optional)
+  int arith_f32;               // 1: MTTKRP on the FP32 matrix cores (FP64 totals), 0: FP64 matrix cores
 };
 size_t batch_lds_bytes();
 void batch_init(hipStream_t s, const BatchArgs &a);
