@@ -323,6 +323,35 @@ def test_als_fp32_stream_and_fused_line_search(h, arith, dims, R):
         assert abs(rel * rel - rel_o * rel_o) < 1e-5
 
 
+def test_time_passes_events_leave_results_unchanged(h):
+    """time_passes brackets the two tensor passes of every sweep with HIP events: the
+    factors, costs and decisions are bitwise those of the untimed run, the order-3 tree
+    reports both passes in every sweep that runs them (the left-node pass from sweep 1, the
+    paired one from the line search on; the right-node pass in every sweep), and the
+    per-sweep times stay within that sweep's wall time."""
+    dims, R = [120, 110, 100], 12
+    X, _ = o.make_tensor(dims, R, seed=4, snr_db=30.0)
+    init = o.gen_factors(dims, R, seed=5)
+    h.set_tensor(X, storage=P.TENSOR_F32)
+    kw = dict(max_iter=10, threshold_error=0.0, mttkrp_form=P.MTTKRP_DIMTREE, arith=P.ARITH_BF16X3, fuse_accel=1)
+    F0, st0, lg0 = h.cpd(R, init, **kw)
+    F1, st1, lg1 = h.cpd(R, init, time_passes=1, **kw)
+    assert st0["f_value"] == st1["f_value"]
+    assert [l["f_accel"] for l in lg0] == [l["f_accel"] for l in lg1]
+    for a, b in zip(F0, F1):
+        assert np.array_equal(a, b)
+    assert all(l["t_pass_ms"] == [0.0, 0.0] for l in lg0)
+    t_prev = 0.0
+    for k, l in enumerate(lg1):
+        left, right = l["t_pass_ms"]
+        say("sweep", k + 1, "passes (ms)", left, right)
+        assert right > 0.0
+        if k + 1 < len(lg1):                 # the last sweep launches no further left-node pass
+            assert left > 0.0
+        assert left + right < 1e3 * (l["t_end"] - t_prev) + 1.0
+        t_prev = l["t_end"]
+
+
 @pytest.mark.parametrize("dims,R,seed,sweeps", [([5, 4, 6, 3, 4], 3, 5, 9), ([4, 3, 5, 3, 4, 2], 3, 6, 10),
                                                 ([3, 4, 5, 3, 2, 3], 2, 7, 10)])
 def test_als_higher_order_standard_form(h, dims, R, seed, sweeps):
