@@ -326,6 +326,59 @@ __global__ __launch_bounds__(256) void k_contract2(const TT *__restrict__ Y, int
   }
 }
 
+// k_contract2's outmode 0 for FP32 nodes with 16-byte loads: thread (rr = tid / 8, c4 = tid % 8)
+// owns row i0 = 32 bx + rr and columns 4 c4 .. 4 c4 + 3 of each 32-column group; each column
+// sums its i1 range in the same order as k_contract2, so the partials are bitwise equal.
+// Rows of Y hold ld floats, ld and the node's offset multiples of 4 (16-byte aligned).
+__global__ __launch_bounds__(256) void k_contract2_r4(const float *__restrict__ Y, int64_t ld, int64_t D0, int64_t D1,
+                                                      const double *__restrict__ F, int ldr, int64_t slen,
+                                                      double *__restrict__ part, const int *sel, int64_t sel_off,
+                                                      const float *__restrict__ Y2, const double *__restrict__ F2) {
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  if (sel) Y += (int64_t)(*sel) * sel_off;
+  if (blockIdx.z) {
+    Y = Y2;
+    F = F2;
+  }
+  const int64_t zs = (int64_t)blockIdx.y * gridDim.z + blockIdx.z;
+  const int rr = threadIdx.x >> 3, c4 = 4 * (threadIdx.x & 7);
+  const int64_t i0 = (int64_t)blockIdx.x * 32 + rr;
+  if (i0 >= D0) return;
+  const int64_t b = (int64_t)blockIdx.y * slen, e = b + slen < D1 ? b + slen : D1;
+  for (int cb = 0; cb < ldr; cb += 32) {
+    const int c = cb + c4;
+    if (c >= ldr) break;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int64_t i1 = b;
+    for (; i1 + 4 <= e; i1 += 4) {
+      f4 y[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) y[u] = *(const f4 *)(Y + (i0 + D0 * (i1 + u)) * ld + c);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const double *f = F + (i1 + u) * ldr + c;
+        a0 = fma((double)y[u][0], f[0], a0);
+        a1 = fma((double)y[u][1], f[1], a1);
+        a2 = fma((double)y[u][2], f[2], a2);
+        a3 = fma((double)y[u][3], f[3], a3);
+      }
+    }
+    for (; i1 < e; ++i1) {
+      const f4 y = *(const f4 *)(Y + (i0 + D0 * i1) * ld + c);
+      const double *f = F + i1 * ldr + c;
+      a0 = fma((double)y[0], f[0], a0);
+      a1 = fma((double)y[1], f[1], a1);
+      a2 = fma((double)y[2], f[2], a2);
+      a3 = fma((double)y[3], f[3], a3);
+    }
+    double *o = part + (zs * D0 + i0) * ldr + c;
+    o[0] = a0;
+    o[1] = a1;
+    o[2] = a2;
+    o[3] = a3;
+  }
+}
+
 __global__ void k_reduce_splits(const double *__restrict__ partials, int splits, int64_t n, double *__restrict__ out) {
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
     double s = 0.0;
@@ -395,6 +448,9 @@ void contract2(hipStream_t s, const double *Y64, const float *Y32, int64_t ld, i
   if (Y64)
     k_contract2<double><<<grid, 256, 0, s>>>(Y64, ld, D0, D1, outmode, F, ldr, R, slen, part, sel, sel_off, nullptr,
                                              nullptr);
+  else if (outmode == 0 && ld % 4 == 0 && sel_off % 4 == 0 && ((uintptr_t)Y32 & 15) == 0)
+    k_contract2_r4<<<dim3((unsigned)((D0 + 31) / 32), (unsigned)splits), 256, 0, s>>>(
+        Y32, ld, D0, D1, F, ldr, slen, part, sel, sel_off, nullptr, nullptr);
   else
     k_contract2<float><<<grid, 256, 0, s>>>(Y32, ld, D0, D1, outmode, F, ldr, R, slen, part, sel, sel_off, nullptr,
                                             nullptr);
@@ -404,6 +460,11 @@ void contract2_pair(hipStream_t s, const float *Ya, const float *Yb, int64_t ld,
                     const double *Fa, const double *Fb, int ldr, int R, int splits, double *part) {
   const int64_t D = outmode == 0 ? D1 : D0;
   const int64_t slen = (D + splits - 1) / splits;
+  if (outmode == 0 && ld % 4 == 0 && ((uintptr_t)Ya & 15) == 0 && ((uintptr_t)Yb & 15) == 0) {
+    k_contract2_r4<<<dim3((unsigned)((D0 + 31) / 32), (unsigned)splits, 2), 256, 0, s>>>(
+        Ya, ld, D0, D1, Fa, ldr, slen, part, nullptr, 0, Yb, Fb);
+    return;
+  }
   dim3 grid(outmode == 0 ? (unsigned)((D0 + 7) / 8) : (unsigned)D1, (unsigned)splits, 2);
   k_contract2<float><<<grid, 256, 0, s>>>(Ya, ld, D0, D1, outmode, Fa, ldr, R, slen, part, nullptr, 0, Yb, Fb);
 }
