@@ -113,23 +113,25 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 template <bool F32>
 __device__ __forceinline__ int dtile_row(int kk, int j) { return F32 ? 4 * kk + j : kk + 4 * j; }
-constexpr int kSteps = 8;                   // k-steps whose tensor loads a lane issues together
+constexpr int kSteps = 12;                  // k-steps whose tensor loads a lane issues together
 
 struct MfmaPart {
   f64x4 a0, a1, b0, b1;                      // slot 0 (first partial band): tiles a0 | a1; slot 1: b0 | b1
   int band0, band1;
 };
 
-template <int MODE, bool F32>
-__device__ __forceinline__ void mttkrp_mfma(const float *X, int I0, int I1, int I2, const double *Fa,
+// X is the problem's tensor laid out with the output mode fastest: element (row, k) at
+// row + Iout k, k = ia + Ia ib (the mode-0 order itself for mode 0; for modes 1 and 2 the
+// copies the batch keeps in (i1, i0, i2) and (i2, i0, i1) order), so the 16 rows a load
+// instruction touches are consecutive floats.
+template <bool F32>
+__device__ __forceinline__ void mttkrp_mfma(const float *X, int Iout, int Ia, int Ib, const double *Fa,
                                             const double *Fb, double *M, int nw, MfmaPart &part) {
   const int t = tid(), lane = t & 63, wave = t >> 6;
   part.band0 = part.band1 = -1;
   if (wave >= nw) return;
-  const int Iout = MODE == 0 ? I0 : MODE == 1 ? I1 : I2;
-  const int Ia = MODE == 0 ? I1 : I0, Ib = MODE == 2 ? I1 : I2;
   const int K = Ia * Ib, KS = (K + 3) / 4, KF = K / 4, NB = (Iout + 31) / 32;
-  const int rs = MODE == 0 ? 1 : MODE == 1 ? I0 : I0 * I1;
+  constexpr int rs = 1;
   const int64_t S = (int64_t)NB * KS;
   const int64_t s0 = (int64_t)wave * S / nw, s1 = (int64_t)(wave + 1) * S / nw;
   const int m = lane & 15, kk = lane >> 4;
@@ -143,9 +145,7 @@ __device__ __forceinline__ void mttkrp_mfma(const float *X, int I0, int I1, int 
     const float *x1 = X + (r1 < Iout ? r1 : Iout - 1) * rs;
     int k = 4 * (int)(s - tb) + kk, ia = k % Ia, ib = k / Ia;
     f64x4 c0 = {0.0, 0.0, 0.0, 0.0}, c1 = {0.0, 0.0, 0.0, 0.0};
-    auto offset = [&](int kx, int iax, int ibx) {
-      return MODE == 0 ? I0 * kx : MODE == 1 ? iax + I0 * I1 * ibx : kx;
-    };
+    auto offset = [&](int kx, int, int) { return Iout * kx; };
     auto advance = [&] {
       k += 4;
       ia += inc_a;
@@ -160,23 +160,30 @@ __device__ __forceinline__ void mttkrp_mfma(const float *X, int I0, int I1, int 
     int n = (int)(ef - s);
     for (; n >= kSteps; n -= kSteps) {
       float xa[kSteps], xb[kSteps];
-      int ias[kSteps], ibs[kSteps];
+      int ka = ia, kb = ib;                            // the group's first (ia, ib), walked again below
 #pragma unroll
       for (int u = 0; u < kSteps; ++u) {               // all tensor loads first
         const int off = offset(k, ia, ib);
         xa[u] = x0[off];
         xb[u] = x1[off];
-        ias[u] = ia;
-        ibs[u] = ib;
         advance();
       }
+      auto knext = [&] {
+        ka += inc_a;
+        kb += inc_b;
+        if (ka >= Ia) {
+          ka -= Ia;
+          ++kb;
+        }
+      };
       if constexpr (F32) {
         f32x4 e0 = {0.f, 0.f, 0.f, 0.f}, e1 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int u = 0; u < kSteps; ++u) {
-          const float bv = (float)(Fa[ias[u] * kBR + m] * Fb[ibs[u] * kBR + m]);
+          const float bv = (float)(Fa[ka * kBR + m] * Fb[kb * kBR + m]);
           e0 = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[u], bv, e0, 0, 0, 0);
           e1 = __builtin_amdgcn_mfma_f32_16x16x4f32(xb[u], bv, e1, 0, 0, 0);
+          knext();
         }
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -186,9 +193,10 @@ __device__ __forceinline__ void mttkrp_mfma(const float *X, int I0, int I1, int 
       } else {
 #pragma unroll
         for (int u = 0; u < kSteps; ++u) {
-          const double bv = Fa[ias[u] * kBR + m] * Fb[ibs[u] * kBR + m];
+          const double bv = Fa[ka * kBR + m] * Fb[kb * kBR + m];
           c0 = __builtin_amdgcn_mfma_f64_16x16x4f64((double)xa[u], bv, c0, 0, 0, 0);
           c1 = __builtin_amdgcn_mfma_f64_16x16x4f64((double)xb[u], bv, c1, 0, 0, 0);
+          knext();
         }
       }
     }
@@ -283,18 +291,18 @@ __device__ __forceinline__ void mttkrp_mfma_reduce(int Iout, int nw, const doubl
 
 // one mode's MTTKRP into S.M with operands S.F[0], S.F[1]: waves [0, nw) compute, `other`
 // runs on the remaining waves meanwhile; P holds 2 x 32 x 16 doubles per wave
-template <int BS, int MODE, bool F32, typename Other>
-__device__ __forceinline__ void mttkrp_block(const float *X, int I0, int I1, int I2, BatchLds &S, double *P, int nw,
+template <int BS, bool F32, typename Other>
+__device__ __forceinline__ void mttkrp_block(const float *X, int Iout, int Ia, int Ib, BatchLds &S, double *P, int nw,
                                              Other &&other) {
   MfmaPart part;
-  if ((int)(tid() >> 6) < nw) mttkrp_mfma<MODE, F32>(X, I0, I1, I2, S.F[0], S.F[1], S.M, nw, part);
+  if ((int)(tid() >> 6) < nw) mttkrp_mfma<F32>(X, Iout, Ia, Ib, S.F[0], S.F[1], S.M, nw, part);
   else {
     part.band0 = part.band1 = -1;
     other();
   }
   mttkrp_mfma_partials<F32>(part, nw, P, S.seg);
   __syncthreads();
-  mttkrp_mfma_reduce<BS>(MODE == 0 ? I0 : MODE == 1 ? I1 : I2, nw, P, S.seg, S.M);
+  mttkrp_mfma_reduce<BS>(Iout, nw, P, S.seg, S.M);
   __syncthreads();
 }
 
@@ -441,10 +449,8 @@ __device__ __forceinline__ void sweep_problem(const BatchArgs &a, int p, int swe
       eig_extremes_wave<kBR>(R, S.eig, amax, amin);
       if (l == 0) glambda_betas(amax, amin, &S.prm, a.max_inner, S.betas);
     };
-    if (n == 0) mttkrp_block<BS, 0, F32>(X, I0, I1, I2, S, P, NW - 1, eig);
-    else if (n == 1) mttkrp_block<BS, 1, F32>(X, I0, I1, I2, S, P, NW - 1, eig);
-    else mttkrp_block<BS, 2, F32>(X, I0, I1, I2, S, P, NW - 1, eig);
     const int In = n == 0 ? I0 : n == 1 ? I1 : I2;
+    mttkrp_block<BS, F32>(X + (int64_t)n * a.xcopy, In, Ia, Ib, S, P, NW - 1, eig);
     const int it = nnls_block<BS, RP>(In, R, S, n == 0 ? A0 : n == 1 ? A1 : A2, a.d1, a.d2, a.max_inner);
     if (t == 0) S.iters[n] = it;
     gram_block<BS>(S.F[0], In, R, S.G[n]);
@@ -510,7 +516,7 @@ __device__ __forceinline__ void sweep_problem(const BatchArgs &a, int p, int swe
       __syncthreads();
       gram_block<BS>(S.F[0], I0, R, S.Gh[0]);
       gram_block<BS>(S.F[1], I1, R, S.Gh[1]);
-      mttkrp_block<BS, 2, F32>(X, I0, I1, I2, S, P, NW, [] {});
+      mttkrp_block<BS, F32>(X + 2 * a.xcopy, I2, I0, I1, S, P, NW, [] {});
       double ma = 0.0, vg = 0.0;
       for (int e = t; e < I2 * kBR; e += BS) ma = fma(S.M[e], Ah2[e], ma);
       if (t < kBR * kBR) vg = (S.Gh[1][t] * S.Gh[0][t]) * S.Gh[2][t];
@@ -576,6 +582,22 @@ __global__ __launch_bounds__(BS, BS == 512 ? 2 : 1) void k_batch_sweep(BatchArgs
   }
 }
 
+// The copies of one problem's tensor with mode 1 / mode 2 fastest, written in their own
+// order: X1[i1 + I1 (i0 + I0 i2)] and X2[i2 + I2 (i0 + I0 i1)] from X[i0 + I0 (i1 + I1 i2)].
+__global__ void k_batch_permute(const float *__restrict__ X, int I0, int I1, int I2, float *__restrict__ X1,
+                                float *__restrict__ X2) {
+  const int64_t n = (int64_t)I0 * I1 * I2;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    if (blockIdx.y == 0) {
+      const int64_t i1 = e % I1, r = e / I1, i0 = r % I0, i2 = r / I0;
+      X1[e] = X[i0 + I0 * (i1 + I1 * i2)];
+    } else {
+      const int64_t i2 = e % I2, r = e / I2, i0 = r % I0, i1 = r / I0;
+      X2[e] = X[i0 + I0 * (i1 + I1 * i2)];
+    }
+  }
+}
+
 __global__ void k_batch_next(int *ctr) {
   if (threadIdx.x == 0) *ctr = *ctr + 1;
 }
@@ -622,5 +644,12 @@ void batch_sweep(hipStream_t s, const BatchArgs &a, int block, int grid) {
 }
 
 void batch_next_sweep(hipStream_t s, int *ctr) { k_batch_next<<<1, 64, 0, s>>>(ctr); }
+
+void batch_permute(hipStream_t s, const float *X, int I0, int I1, int I2, float *X1, float *X2) {
+  const int64_t n = (int64_t)I0 * I1 * I2;
+  int64_t g = (n + 255) / 256;
+  if (g > 2048) g = 2048;
+  k_batch_permute<<<dim3((unsigned)g, 2), 256, 0, s>>>(X, I0, I1, I2, X1, X2);
+}
 
 }  // namespace ptn

@@ -80,11 +80,14 @@ struct BatchArgs {
   int pow_stride;
   BatchLog *log;               // [nprob][max_iter] (optional)
   int arith_f32;               // 1: MTTKRP on the FP32 matrix cores (FP64 totals), 0: FP64 matrix cores
+  int64_t xcopy;               // X + n xcopy: the tensors with mode n fastest ((i1, i0, i2), (i2, i0, i1) for n = 1, 2)
 };
 size_t batch_lds_bytes();
 void batch_init(hipStream_t s, const BatchArgs &a);
 void batch_sweep(hipStream_t s, const BatchArgs &a, int block, int grid);
 void batch_next_sweep(hipStream_t s, int *ctr);
+// the mode-1 / mode-2 copies of one problem's tensor (BatchArgs::xcopy)
+void batch_permute(hipStream_t s, const float *X, int I0, int I1, int I2, float *X1, float *X2);
 
 // ---- update_factor's unconstrained and orthogonality branches (constraint_kernels.hip) ----
 void lu_inverse(hipStream_t s, int R, const double *V, double *Vi);            // NaN when singular
