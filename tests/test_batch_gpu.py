@@ -94,7 +94,7 @@ def test_batch_parity_vs_oracle(h):
 
 def test_batch_fp32_matrix_cores_vs_fp64(h):
     """arith = F32 runs the batch MTTKRPs on the FP32 matrix cores (FP32 products of the FP32
-    tensor and the FP32-rounded Khatri-Rao values, FP64 totals every 32 products); every
+    tensor and the FP32-rounded Khatri-Rao values, FP64 totals every 12 k-steps); every
     other step stays FP64.  Each problem's final relative cost agrees with the FP64 run and
     with the oracle to 2e-6 absolute (measured on the 512-problem batch after 100 sweeps:
     at most 7.2e-7, median 9e-9); both block sizes give the same bits on different grids."""
