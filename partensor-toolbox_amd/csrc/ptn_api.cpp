@@ -1129,9 +1129,8 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
       const bool cap = it >= (int)o.max_iter;
       const bool step_now = !cap && o.acceleration && it > 1;
       // with the paired pass the sweep's cost is needed first by the line-search
-      // decision, so it runs on the side stream beside Normalize and the next pass
-      // (Normalize leaves the last factor and its Gram as they are unless it is the
-      // weight factor)
+      // decision, so it runs on the side stream beside the next pass (Normalize leaves
+      // the last factor and its Gram as they are unless it is the weight factor)
       const bool side_cost = step_now && pair && !(o.normalization && !all_orth && wf == last);
       for (int n = 0; n < N; ++n) {
         if (!(n == 0 && prepped0)) {
@@ -1144,11 +1143,8 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
         if (!(n == 0 && prepped0)) join();   // else joined after the line-search pass
         update_factor(w, n, o, sel[n]);
         if (n == 0 && tree && w.mfma32 && N == 3) right_node_early(w, w.fac);
-        if (n == last) {
-          if (side_cost) fork();
-          cost_gram(side_cost ? ss : ms, w.dims[last] * w.ldr, w.M, w.F(last), R, sel[last].V, w.G(last), w.scal.p,
-                    w.scal.p + 1, 0);
-        }
+        if (n == last && !side_cost)
+          cost_gram(ms, w.dims[last] * w.ldr, w.M, w.F(last), R, sel[last].V, w.G(last), w.scal.p, w.scal.p + 1, 0);
       }
       prepped0 = false;
       ExportList ex{};                 // the sweep's readback: NNLS parameters and the cost
@@ -1193,6 +1189,11 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
           } else {
             mttkrp_standard(w, w.acc, last, w.M);
           }
+          // the sweep's cost (Normalize and the step leave M, A_last, V_last and G_last as
+          // they are when the weight factor is not the last; the decision reads it after
+          // the join)
+          if (side_cost)
+            cost_gram(ss, w.dims[last] * w.ldr, w.M, w.F(last), R, sel[last].V, w.G(last), w.scal.p, w.scal.p + 1, 0);
           if (pair) prep_mode(w, w.grams, 0, tree, slot(w, 0), o.max_inner, ss);
           for (int k = 0; k < N; ++k)
             gram(ss, w.dims[k], R, w.ldr, w.Fb(w.acc, k), w.agr.p + (int64_t)k * R * R, w.gpart_side);
