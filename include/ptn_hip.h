@@ -189,7 +189,7 @@ ptn_status ptn_batch_init_factors(ptn_handle h, uint64_t seed, double *packed_ou
 ptn_status ptn_batch_get_tensor(ptn_handle h, int p, double *X_out);
 /* ALS on every problem of the batch with the options of ptn_cpd (mttkrp_form and
    fuse_accel are not consulted; arith: PTN_ARITH_F64 runs the MTTKRPs on the FP64 matrix
-   cores, PTN_ARITH_F32 on the FP32 matrix cores with FP64 totals every 32 products, all
+   cores, PTN_ARITH_F32 on the FP32 matrix cores with FP64 totals every 12 k-steps, all
    else FP64 either way).  init_packed / out_packed: factors packed per problem,
    problems back to back; res: nprob records; log: NULL or [nprob][opt->max_iter].
    block / grid: launch shape of the sweep kernel (256 or 512 threads), or both 0 to let
