@@ -215,7 +215,9 @@ struct RowSpec {
 template <typename TT>
 __global__ __launch_bounds__(256) void k_contract_rows(const TT *__restrict__ T, int64_t ldt, RowSpec sp, int RP,
                                                        int ldr, int64_t cchunk, int64_t Cother,
-                                                       double *__restrict__ partials) {
+                                                       double *__restrict__ partials, const int *sel,
+                                                       int64_t sel_off) {
+  if (sel) T += (int64_t)(*sel) * sel_off;              // node chosen on the device (line search)
   __shared__ double red[256];
   const int r = threadIdx.x % RP, grp = threadIdx.x / RP, ngrp = 256 / RP;
   const uint32_t i = blockIdx.x;
@@ -421,7 +423,7 @@ void gemm_ck_f64(hipStream_t s, const double *x64, const float *x32, int64_t L, 
 }
 
 void contract_rows(hipStream_t s, const double *T64, const float *T32, int64_t ldt, const KrSpec &spec, int outmode,
-                   int ldr, int splits, double *partials) {
+                   int ldr, int splits, double *partials, const int *sel, int64_t sel_off) {
   RowSpec sp{};
   sp.nm = spec.nm;
   sp.outmode = outmode;
@@ -436,8 +438,8 @@ void contract_rows(hipStream_t s, const double *T64, const float *T32, int64_t l
   int RP = ldr;
   int64_t cchunk = (Cother + splits - 1) / splits;
   dim3 grid((unsigned)spec.dim[outmode], (unsigned)splits);
-  if (T64) k_contract_rows<double><<<grid, 256, 0, s>>>(T64, ldt, sp, RP, ldr, cchunk, Cother, partials);
-  else     k_contract_rows<float><<<grid, 256, 0, s>>>(T32, ldt, sp, RP, ldr, cchunk, Cother, partials);
+  if (T64) k_contract_rows<double><<<grid, 256, 0, s>>>(T64, ldt, sp, RP, ldr, cchunk, Cother, partials, sel, sel_off);
+  else     k_contract_rows<float><<<grid, 256, 0, s>>>(T32, ldt, sp, RP, ldr, cchunk, Cother, partials, sel, sel_off);
 }
 
 void contract2(hipStream_t s, const double *Y64, const float *Y32, int64_t ld, int64_t D0, int64_t D1, int outmode,

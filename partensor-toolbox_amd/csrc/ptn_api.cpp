@@ -501,10 +501,9 @@ void contract_node(Work &w, const Node &nd, const KrSpec &sp, int outmode, doubl
     if (S > 1) reduce_splits(s, w.part, S, In * w.ldr, M);
     return;
   }
-  REQUIRE(nd.sel == nullptr, PTN_ERR_INVALID, "device-selected node needs two digits");
   const int S = pick_splits(In, Cother / 64, 64);
   REQUIRE(S * In * w.ldr <= w.part_cap, PTN_ERR_INVALID, "partial buffer too small");
-  contract_rows(s, nd.p64, nd.p32, nd.ld, sp, outmode, w.ldr, S, w.part);
+  contract_rows(s, nd.p64, nd.p32, nd.ld, sp, outmode, w.ldr, S, w.part, nd.sel, nd.sel_off);
   reduce_splits(s, w.part, S, In * w.ldr, M);
 }
 
@@ -643,10 +642,15 @@ void left_node_pair(Work &w, double *cur, double *acc) {
   }
   const KrSpec la = kr_range(w, acc, 0, hf), lc = kr_range(w, cur, 0, hf);
   const int64_t D0 = la.dim[0], D1 = la.dim[1];
-  const int S2 = pick_splits((D0 + 7) / 8, D1 / 16, 64);   // contract_node's split of outmode 0
-  REQUIRE(2 * S2 * D0 * w.ldr <= w.part_cap, PTN_ERR_INVALID, "partial buffer too small");
-  contract2_pair(s, w.YL2.p + w.rpf, w.YL2.p, width, D0, D1, 0, la.fac[1], lc.fac[1], w.ldr, w.R, S2, w.part);
-  reduce_splits(s, w.part, S2, 2 * D0 * w.ldr, w.M2);
+  if (la.nm == 2) {
+    const int S2 = pick_splits((D0 + 7) / 8, D1 / 16, 64);   // contract_node's split of outmode 0
+    REQUIRE(2 * S2 * D0 * w.ldr <= w.part_cap, PTN_ERR_INVALID, "partial buffer too small");
+    contract2_pair(s, w.YL2.p + w.rpf, w.YL2.p, width, D0, D1, 0, la.fac[1], lc.fac[1], w.ldr, w.R, S2, w.part);
+    reduce_splits(s, w.part, S2, 2 * D0 * w.ldr, w.M2);
+  } else {                                   // three or more digits (orders 5 .. 8): one contraction per half
+    contract_node(w, Node{nullptr, w.YL2.p + w.rpf, width}, la, 0, w.M2);
+    contract_node(w, Node{nullptr, w.YL2.p, width}, lc, 0, w.M2 + D0 * w.ldr);
+  }
   w.m0_ready = true;
 }
 
@@ -1035,7 +1039,6 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
     const bool pair = o.fuse_accel != 0;
     REQUIRE(!pair || (tree && (o.arith == PTN_ARITH_F32 || o.arith == PTN_ARITH_BF16X3)), PTN_ERR_UNSUPPORTED,
             "fuse_accel needs the dimension tree with F32 or BF16X3 contractions");
-    REQUIRE(!pair || h->N <= 4, PTN_ERR_UNSUPPORTED, "fuse_accel is built for orders 3 and 4");
     REQUIRE(!o.warm_start || (h->mlp && h->mlp->ready), PTN_ERR_STATE,
             "warm_start needs a network (ptn_mlp_init or ptn_mlp_set_params)");
 

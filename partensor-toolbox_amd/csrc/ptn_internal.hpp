@@ -157,8 +157,10 @@ void gemm_ck_f64(hipStream_t s, const double *x64, const float *x32, int64_t L, 
 // out[i][r] (+)= sum over rows t of T whose output digit is i of T[t][r] * prod of the
 // other digits' factor rows.  `t` enumerates spec modes lowest fastest; `outmode` is the
 // position in spec of the output index.  Rows of T are ldt apart; T may be FP64 or FP32.
+// sel (optional, device): the node starts at T + (*sel) * sel_off
 void contract_rows(hipStream_t s, const double *T64, const float *T32, int64_t ldt, const KrSpec &spec,
-                   int outmode, int ldr, int splits, double *partials);
+                   int outmode, int ldr, int splits, double *partials, const int *sel = nullptr,
+                   int64_t sel_off = 0);
 void reduce_splits(hipStream_t s, const double *partials, int splits, int64_t n, double *out);
 // Two-digit node Y (rows i0 + D0 i1, ld values per row) contracted against the other
 // digit's factor F: part[split][i][ldr] for the kept digit i (outmode 0: i0, 1: i1).

@@ -357,8 +357,7 @@ def test_time_passes_events_leave_results_unchanged(h):
 def test_als_higher_order_standard_form(h, dims, R, seed, sweeps):
     """Orders 5 and 6 through the per-mode form.  The sweep counts stop before the runs
     converge to where f and the accelerated cost tie to ~1e-14 and the decision rests on
-    the last bits (smallest oracle margin here >= 1.4e-4 relative).  The fused line-search
-    pass is built for orders 3 and 4 only."""
+    the last bits (smallest oracle margin here >= 1.4e-4 relative)."""
     say("ALS order", len(dims), dims, R)
     X, _ = o.make_tensor(dims, R, seed=seed, snr_db=25.0)
     init = o.gen_factors(dims, R, seed=seed + 1)
@@ -367,8 +366,27 @@ def test_als_higher_order_standard_form(h, dims, R, seed, sweeps):
     Fg, stg, logg = h.cpd(R, init, **kw)
     Fo, sto, logo = o.cpd(X, R, init, mttkrp_form=o.MTTKRP_STANDARD, cost_form=o.COST_GRAM, **kw)
     _compare_runs(Fg, stg, logg, Fo, sto, logo, 1e-10)
-    with pytest.raises(P.PtnError):
-        h.cpd(R, init, mttkrp_form=P.MTTKRP_DIMTREE, arith=P.ARITH_F32, fuse_accel=1, **kw)
+
+
+def _fused_equals_unfused(h, R, init, kw):
+    """The tree with FP32 contractions, with and without the line-search pass fused into
+    the next left node: while the two runs take the same acceleration decisions they are
+    bitwise equal apart from f_accel (its fit term is summed in another order)."""
+    kt = dict(kw, mttkrp_form=P.MTTKRP_DIMTREE, arith=P.ARITH_F32)
+    F0, st0, lg0 = h.cpd(R, init, fuse_accel=0, **kt)
+    F1, st1, lg1 = h.cpd(R, init, fuse_accel=1, **kt)
+    acc0 = [l["accel_accepted"] for l in lg0]
+    acc1 = [l["accel_accepted"] for l in lg1]
+    assert [l["accel_tried"] for l in lg0] == [l["accel_tried"] for l in lg1]
+    say("fused line search: accept unfused", acc0, "fused", acc1)
+    if acc0 == acc1:
+        assert [l["nnls_iters"] for l in lg0] == [l["nnls_iters"] for l in lg1]
+        assert st0["f_value"] == st1["f_value"]
+        for a, b in zip(F0, F1):
+            assert np.array_equal(a, b)
+    else:
+        k = next(i for i, (a, b) in enumerate(zip(acc0, acc1)) if a != b)
+        assert abs(lg0[k]["f_value"] - lg0[k]["f_accel"]) <= 1e-6 * lg0[k]["f_value"], k
 
 
 @pytest.mark.parametrize("cons", [[0, 0, 0], [2, 1, 1], [1, 0, 2], [1, 4, 1], [0, 2, 1, 3]])
@@ -405,8 +423,9 @@ def test_als_other_constraints(h, cons):
                                     ([3, 2, 3, 2, 2, 3, 2, 3], 2)])
 def test_dimtree_higher_orders(h, dims, R):
     """Orders 5-8 in the tree: the root's children (modes [0, ceil(N/2)) and the rest), each
-    one pass over X; every MTTKRP against the oracle's, FP64 and FP32 stream; then ALS in the
-    tree form against the oracle's standard form."""
+    one pass over X; every MTTKRP against the oracle's, FP64 and FP32 stream; the fused
+    line-search pass against the unfused one; then ALS in the tree form against the
+    oracle's standard form."""
     say("tree order", len(dims), dims)
     rng = np.random.default_rng(len(dims))
     X = np.asfortranarray(rng.standard_normal(dims))
@@ -420,8 +439,9 @@ def test_dimtree_higher_orders(h, dims, R):
         assert P.rel_err(D32[mode], ref) < 1e-6, mode
     Xc, _ = o.make_tensor(dims, R, seed=len(dims) + 20, snr_db=30.0)
     init = o.gen_factors(dims, R, seed=len(dims) + 21)
-    h.set_tensor(Xc)
+    h.set_tensor(Xc, storage=P.TENSOR_BOTH)
     kw = dict(max_iter=8, threshold_error=0.0)
+    _fused_equals_unfused(h, R, init, kw)
     Fg, stg, logg = h.cpd(R, init, mttkrp_form=P.MTTKRP_DIMTREE, **kw)
     Fo, sto, logo = o.cpd(Xc, R, init, cost_form=o.COST_GRAM, **kw)
     tie = _first_accel_tie(logg, logo)
