@@ -1074,8 +1074,11 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
     HIPCHK(hipMemcpyAsync(w.scal.p, &r.frob_sq, sizeof(double), hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
 
-    // Two streams: the side stream forms V_n, its eigenvalues and the momentum
-    // coefficients of mode n while the main stream contracts the tensor for mode n.
+    // The side stream forms V_n, its eigenvalues and the momentum coefficients of mode n
+    // while the main stream contracts the tensor for mode n; during the line-search pass
+    // it also forms the accelerated factors' Grams, V_hat, the sweep's cost and both
+    // mode-0 set-ups of the next sweep.  The aux stream forms the order-3 tree's right node
+    // (right_node_early).  All three run at the default priority.
     hipStream_t ms = h->stream, ss = h->side;
     auto fork = [&] {      // side waits for the main stream's work so far
       HIPCHK(hipEventRecord(h->ev[0], ms));
@@ -1087,8 +1090,9 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
     };
     ModeIn sel[kMaxOrder];
     bool prepped0 = false;
-    // The host enqueues sweep it + 1 before it reads sweep it back (pinned buffers, one
-    // event per sweep), so the device never waits for the host.  The line-search
+    // The host enqueues sweep it + 1 before it reads sweep it back (page-locked buffers
+    // the kernels write directly, one event per sweep), so the device never waits for
+    // the host.  The line-search
     // decision and its counters live on the device (flags[0..3]); a sweep that ends the
     // run on the threshold is found one sweep late, and the factors come from its
     // snapshot.
