@@ -112,10 +112,10 @@ __device__ __forceinline__ void mma_step(const float (&x)[8], const bf16x8 &bhi,
 // multiplies; occupancy hides the load latency.  The 64 x 32 result of each column tile
 // leaves through LDS as 16-byte non-temporal row stores: scattered 4-byte stores of the
 // MFMA layout cost several times their bytes next to the tensor stream.  CAP (bf16x3,
-// launches without a k split): the staging area is padded so that at most two blocks
-// share a CU; the strided tensor stream of an 800^3 left node runs 3-10% faster with 8
-// waves per CU than with the 20 the registers allow, while the k-split order-4 node
-// (200^4) runs 4% slower, so split launches keep the full occupancy.
+// 32 or 64 columns): the staging area is padded so that at most two blocks share a CU;
+// the strided tensor stream runs faster with 8 waves per CU than with the 12-20 the
+// registers allow (800^3 left node, 32 columns: 3-10%; 200^4 order-4 node, 64 columns
+// split 13 ways: 1.113 ms against 1.234 ms).
 // ---------------------------------------------------------------------------
 template <int MODE, int NT, bool CAP>
 __global__ __launch_bounds__(256) void k_gemm_cm_f32(const float *__restrict__ X, int64_t M, int64_t K,
@@ -517,9 +517,8 @@ void gemm_cm_f32(hipStream_t s, GemmMode mode, const float *x32, int64_t M, int6
 #define PTN_CM32(MODEV, NTV, CAPV) \
   k_gemm_cm_f32<MODEV, NTV, CAPV><<<grid, 256, 0, s>>>(x32, M, K, kchunk, h, l, wf, out)
   if (mode == kGemmBf16x3) {
-    const bool cap = splits == 1;
-    if (RPF == 32) { if (cap) PTN_CM32(kGemmBf16x3, 1, true); else PTN_CM32(kGemmBf16x3, 1, false); }
-    else if (RPF == 64) { if (cap) PTN_CM32(kGemmBf16x3, 2, true); else PTN_CM32(kGemmBf16x3, 2, false); }
+    if (RPF == 32) PTN_CM32(kGemmBf16x3, 1, true);
+    else if (RPF == 64) PTN_CM32(kGemmBf16x3, 2, true);
     else PTN_CM32(kGemmBf16x3, 4, false);
   } else {
     if (RPF == 32) PTN_CM32(kGemmFp32, 1, false);

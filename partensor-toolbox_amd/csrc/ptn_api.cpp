@@ -399,6 +399,13 @@ int pick_splits(int64_t blocks_per_split, int64_t work_per_split_unit, int64_t m
   return (int)s;
 }
 
+// k splits of a strided contraction (gemm_cm) of Mrows rows over K.  bf16x3 launches run
+// at most two blocks per CU (512 on the chip), so the split count is rounded down to give
+// whole waves of 512 blocks: 200^4 at R = 64, 157 row blocks, 13 splits = 2041 blocks
+// (1.145 ms node with its reduction) against 14 = 2198 blocks (1.27 ms, a fifth wave
+// 30% full).
+int pick_splits_cm(const Work &w, int64_t Mrows, int64_t K);
+
 KrSpec kr_of(Work &w, double *base, std::initializer_list<int> modes) {
   KrSpec k{};
   for (int m : modes) {
@@ -422,6 +429,15 @@ KrSpec kr_range(Work &w, double *base, int lo, int hi) {
 
 // out[m][r] = sum_k X[m + M k] W(k)[r]: into dst (S == 1) or via split partials.
 // timed >= 0: the contraction kernel is bracketed by the events of pass `timed`
+int pick_splits_cm(const Work &w, int64_t Mrows, int64_t K) {
+  const int64_t blocks = (Mrows + 255) / 256;
+  if (!(w.mfma32 && w.arith == PTN_ARITH_BF16X3)) return pick_splits(blocks, K / 256, 64);
+  int64_t s = 2048 / blocks;
+  if (s > 64) s = 64;
+  if (s > K / 256) s = K / 256;
+  return s < 1 ? 1 : (int)s;
+}
+
 Node node_cm(Work &w, const KrSpec &k, int64_t Mrows, int64_t K, int S, double *dst64, float *dst32,
              int timed = -1) {
   ptn_handle h = w.h;
@@ -583,7 +599,7 @@ void mttkrp_tree(Work &w, double *base, int mode, double *M) {
   const int hf = w.half();
   const int64_t Ml = w.left_rows(), Kl = w.total / Ml;
   if (mode == 0 && !w.left_ready) {
-    const int S = pick_splits((Ml + 255) / 256, Kl / 256, 64);
+    const int S = pick_splits_cm(w, Ml, Kl);
     keep(w.left, node_cm(w, kr_range(w, base, hf, N), Ml, Kl, S, w.YL.p, w.YL32.p, 0));
   }
   w.left_ready = false;
@@ -629,7 +645,7 @@ void left_node_pair(Work &w, double *cur, double *acc) {
   const int64_t Mrows = w.left_rows(), K = w.total / Mrows;
   const int width = 2 * w.rpf;
   w.pack(k1, K, 0, &k2);
-  const int S = n3 ? 1 : pick_splits((Mrows + 255) / 256, K / 256, 64);
+  const int S = n3 ? 1 : pick_splits_cm(w, Mrows, K);
   w.pass_mark(0, false, s);
   if (S == 1) {
     gemm_cm_f32(s, w.gmode(), w.x32(), Mrows, K, width, 1, w.whi, w.wlo, w.wf, w.YL2);
@@ -1472,7 +1488,7 @@ ptn_status ptn_time_contraction(ptn_handle h, int kernel, int R, const double *f
     const bool fused = kernel == 1 && w.mfma32 && w.N == 3;
     const bool n4 = w.N == 4;
     // order 4: the fused GEMM + TTV node X_(0,1) (A_3 (.) A_2), split over k and reduced
-    const int S4 = n4 ? pick_splits((d[0] * d[1] + 255) / 256, d[2] * d[3] / 256, 64) : 1;
+    const int S4 = n4 ? pick_splits_cm(w, d[0] * d[1], d[2] * d[3]) : 1;
     if (w.mfma32) {
       if (kernel == 0) w.pack(n4 ? kr_of(w, w.fac, {2, 3}) : kr_of(w, w.fac, {2}), n4 ? d[2] * d[3] : d[2], 0);
       else if (kernel == 1) w.pack(kr_of(w, w.fac, {0}), d[0], 1);
