@@ -115,6 +115,17 @@ template <bool F32>
 __device__ __forceinline__ int dtile_row(int kk, int j) { return F32 ? 4 * kk + j : kk + 4 * j; }
 constexpr int kSteps = 12;                  // k-steps whose tensor loads a lane issues together
 
+// A raw buffer over a problem's tensor (bytes), its base in scalar registers: the group
+// loads below take the lane's offset in one register and the step's in a scalar one, so
+// the group's loads need no 64-bit address registers.  Reads past `bytes` return 0.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t tensor_rsrc(const float *X, int bytes) {
+  const uint64_t a = (uint64_t)X;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+  return __builtin_amdgcn_make_buffer_rsrc((void *)(((uint64_t)hi << 32) | lo), (short)0,
+                                           __builtin_amdgcn_readfirstlane(bytes), 0x00020000);
+}
+
 struct MfmaPart {
   f64x4 a0, a1, b0, b1;                      // slot 0 (first partial band): tiles a0 | a1; slot 1: b0 | b1
   int band0, band1;
@@ -136,13 +147,16 @@ __device__ __forceinline__ void mttkrp_mfma(const float *X, int Iout, int Ia, in
   const int64_t s0 = (int64_t)wave * S / nw, s1 = (int64_t)(wave + 1) * S / nw;
   const int m = lane & 15, kk = lane >> 4;
   const int inc_a = 4 % Ia, inc_b = 4 / Ia;      // k += 4 as (ia, ib) += (inc_a, inc_b), one wrap at most
+  const __amdgpu_buffer_rsrc_t rx = tensor_rsrc(X, Iout * K * 4);
+  const int step_bytes = __builtin_amdgcn_readfirstlane(16 * Iout);   // 4 k of one row apart
   int slot = 0;
   for (int64_t s = s0; s < s1;) {
     const int band = (int)(s / KS);
     const int64_t tb = (int64_t)band * KS, te = tb + KS, e = te < s1 ? te : s1;
     const int r0 = band * 32 + m, r1 = r0 + 16;
-    const float *x0 = X + (r0 < Iout ? r0 : Iout - 1) * rs;
-    const float *x1 = X + (r1 < Iout ? r1 : Iout - 1) * rs;
+    const int r0c = r0 < Iout ? r0 : Iout - 1, r1c = r1 < Iout ? r1 : Iout - 1;
+    const float *x0 = X + r0c * rs;
+    const float *x1 = X + r1c * rs;
     int k = 4 * (int)(s - tb) + kk, ia = k % Ia, ib = k / Ia;
     f64x4 c0 = {0.0, 0.0, 0.0, 0.0}, c1 = {0.0, 0.0, 0.0, 0.0};
     auto offset = [&](int kx, int, int) { return Iout * kx; };
@@ -161,11 +175,11 @@ __device__ __forceinline__ void mttkrp_mfma(const float *X, int Iout, int Ia, in
     for (; n >= kSteps; n -= kSteps) {
       float xa[kSteps], xb[kSteps];
       int ka = ia, kb = ib;                            // the group's first (ia, ib), walked again below
+      const int vo0 = (r0c + Iout * k) * 4, vo1 = (r1c + Iout * k) * 4;   // bytes of the group's first step
 #pragma unroll
       for (int u = 0; u < kSteps; ++u) {               // all tensor loads first
-        const int off = offset(k, ia, ib);
-        xa[u] = x0[off];
-        xb[u] = x1[off];
+        xa[u] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rx, vo0, step_bytes * u, 0));
+        xb[u] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rx, vo1, step_bytes * u, 0));
         advance();
       }
       auto knext = [&] {
