@@ -104,7 +104,8 @@ __device__ __forceinline__ void gram_block(const double *A, int I, int R, double
 // row and are not stored.  The bands x KS steps are split evenly over the nw waves
 // (stream-K): a wave stores the bands its range covers whole straight into M and keeps the
 // (at most two) partial bands in registers; mttkrp_mfma_partials / mttkrp_mfma_reduce add
-// them up in wave order (a fixed order: the result does not depend on the grid).
+// them up in wave order (a fixed order: the result does not depend on the grid).  LAST_Q < 4:
+// the last wave takes LAST_Q quarters of a share (it has other work first), the others a share.
 // F32: v_mfma_f32_16x16x4f32 on the FP32 tensor values and the FP32-rounded Khatri-Rao
 // values (twice the FP64 matrix-core rate); each group of kSteps steps accumulates in FP32
 // and is then added to the FP64 totals.  Its D layout is lane l, register j -> row
@@ -135,7 +136,7 @@ struct MfmaPart {
 // row + Iout k, k = ia + Ia ib (the mode-0 order itself for mode 0; for modes 1 and 2 the
 // copies the batch keeps in (i1, i0, i2) and (i2, i0, i1) order), so the 16 rows a load
 // instruction touches are consecutive floats.
-template <bool F32>
+template <bool F32, int LAST_Q>
 __device__ __forceinline__ void mttkrp_mfma(const float *X, int Iout, int Ia, int Ib, const double *Fa,
                                             const double *Fb, double *M, int nw, MfmaPart &part) {
   const int t = tid(), lane = t & 63, wave = t >> 6;
@@ -144,7 +145,15 @@ __device__ __forceinline__ void mttkrp_mfma(const float *X, int Iout, int Ia, in
   const int K = Ia * Ib, KS = (K + 3) / 4, KF = K / 4, NB = (Iout + 31) / 32;
   constexpr int rs = 1;
   const int64_t S = (int64_t)NB * KS;
-  const int64_t s0 = (int64_t)wave * S / nw, s1 = (int64_t)(wave + 1) * S / nw;
+  int64_t s0, s1;
+  if constexpr (LAST_Q == 4) {
+    s0 = (int64_t)wave * S / nw;
+    s1 = (int64_t)(wave + 1) * S / nw;
+  } else {
+    const int64_t den = 4 * (nw - 1) + LAST_Q;
+    s0 = 4 * wave * S / den;
+    s1 = wave + 1 >= nw ? S : 4 * (wave + 1) * S / den;
+  }
   const int m = lane & 15, kk = lane >> 4;
   const int inc_a = 4 % Ia, inc_b = 4 / Ia;      // k += 4 as (ia, ib) += (inc_a, inc_b), one wrap at most
   const __amdgpu_buffer_rsrc_t rx = tensor_rsrc(X, Iout * K * 4);
@@ -303,16 +312,22 @@ __device__ __forceinline__ void mttkrp_mfma_reduce(int Iout, int nw, const doubl
   }
 }
 
-// one mode's MTTKRP into S.M with operands S.F[0], S.F[1]: waves [0, nw) compute, `other`
-// runs on the remaining waves meanwhile; P holds 2 x 32 x 16 doubles per wave
-template <int BS, bool F32, typename Other>
+// one mode's MTTKRP into S.M with operands S.F[0], S.F[1] on waves [0, nw).  LAST_Q = 4: the
+// waves past them run `other` meanwhile; LAST_Q < 4: the last of the nw waves runs `other`
+// first and then LAST_Q quarters of a share.  P holds 2 x 32 x 16 doubles per wave.
+template <int BS, bool F32, int LAST_Q, typename Other>
 __device__ __forceinline__ void mttkrp_block(const float *X, int Iout, int Ia, int Ib, BatchLds &S, double *P, int nw,
                                              Other &&other) {
   MfmaPart part;
-  if ((int)(tid() >> 6) < nw) mttkrp_mfma<F32>(X, Iout, Ia, Ib, S.F[0], S.F[1], S.M, nw, part);
-  else {
-    part.band0 = part.band1 = -1;
-    other();
+  if constexpr (LAST_Q == 4) {
+    if ((int)(tid() >> 6) < nw) mttkrp_mfma<F32, 4>(X, Iout, Ia, Ib, S.F[0], S.F[1], S.M, nw, part);
+    else {
+      part.band0 = part.band1 = -1;
+      other();
+    }
+  } else {
+    if ((int)(tid() >> 6) == nw - 1) other();
+    mttkrp_mfma<F32, LAST_Q>(X, Iout, Ia, Ib, S.F[0], S.F[1], S.M, nw, part);
   }
   mttkrp_mfma_partials<F32>(part, nw, P, S.seg);
   __syncthreads();
@@ -434,7 +449,7 @@ __device__ __forceinline__ int nnls_block(int I, int R, BatchLds &S, double *Ag,
 // one sweep of one problem
 template <int BS, int RP, bool F32>
 __device__ __forceinline__ void sweep_problem(const BatchArgs &a, int p, int sweep, BatchLds &S, double *P) {
-  constexpr int NW = BS / 64;                  // the last wave solves the eigenproblem meanwhile
+  constexpr int NW = BS / 64;                  // the last wave solves the eigenproblem first
   const BatchDesc &d = a.desc[p];
   BatchProb &st = a.st[p];
   if (st.done) return;                                    // stopped on the threshold earlier
@@ -464,7 +479,11 @@ __device__ __forceinline__ void sweep_problem(const BatchArgs &a, int p, int swe
       if (l == 0) glambda_betas(amax, amin, &S.prm, a.max_inner, S.betas);
     };
     const int In = n == 0 ? I0 : n == 1 ? I1 : I2;
-    mttkrp_block<BS, F32>(X + (int64_t)n * a.xcopy, In, Ia, Ib, S, P, NW - 1, eig);
+    // FP64 matrix cores: the eigen wave (R <= 16: microseconds) then takes 3/4 of an MTTKRP
+    // share (1.76 -> 1.69 ms per 512-problem sweep); FP32 ones: the other seven waves split
+    // the MTTKRP (the same share made that sweep 1.49 -> 1.53 ms)
+    if constexpr (F32) mttkrp_block<BS, F32, 4>(X + (int64_t)n * a.xcopy, In, Ia, Ib, S, P, NW - 1, eig);
+    else mttkrp_block<BS, F32, 3>(X + (int64_t)n * a.xcopy, In, Ia, Ib, S, P, NW, eig);
     const int it = nnls_block<BS, RP>(In, R, S, n == 0 ? A0 : n == 1 ? A1 : A2, a.d1, a.d2, a.max_inner);
     if (t == 0) S.iters[n] = it;
     gram_block<BS>(S.F[0], In, R, S.G[n]);
@@ -530,7 +549,7 @@ __device__ __forceinline__ void sweep_problem(const BatchArgs &a, int p, int swe
       __syncthreads();
       gram_block<BS>(S.F[0], I0, R, S.Gh[0]);
       gram_block<BS>(S.F[1], I1, R, S.Gh[1]);
-      mttkrp_block<BS, F32>(X + 2 * a.xcopy, I2, I0, I1, S, P, NW, [] {});
+      mttkrp_block<BS, F32, 4>(X + 2 * a.xcopy, I2, I0, I1, S, P, NW, [] {});
       double ma = 0.0, vg = 0.0;
       for (int e = t; e < I2 * kBR; e += BS) ma = fma(S.M[e], Ah2[e], ma);
       if (t < kBR * kBR) vg = (S.Gh[1][t] * S.Gh[0][t]) * S.Gh[2][t];
