@@ -117,7 +117,19 @@ __device__ __forceinline__ void mma_step(const float (&x)[8], const bf16x8 &bhi,
 // registers allow (800^3 left node, 32 columns: 3-10%; 200^4 order-4 node, 64 columns
 // split 13 ways: 1.113 ms against 1.234 ms).
 // ---------------------------------------------------------------------------
-template <int MODE, int NT, bool CAP>
+// A raw buffer over `bytes` bytes from p (p uniform; its halves taken to scalar registers)
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_rsrc(const float *p, uint32_t bytes) {
+  const uint64_t a = (uint64_t)p;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+  return __builtin_amdgcn_make_buffer_rsrc((void *)(((uint64_t)hi << 32) | lo), (short)0,
+                                           __builtin_amdgcn_readfirstlane(bytes), 0x00020000);
+}
+__device__ __forceinline__ float load_nt(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 2));   // 2: nt
+}
+
+template <int MODE, int NT, bool CAP, bool BUF>
 __global__ __launch_bounds__(256) void k_gemm_cm_f32(const float *__restrict__ X, int64_t M, int64_t K,
                                                      int64_t kchunk, const bf16x8 *__restrict__ whi,
                                                      const bf16x8 *__restrict__ wlo, const float *__restrict__ wf,
@@ -143,9 +155,10 @@ __global__ __launch_bounds__(256) void k_gemm_cm_f32(const float *__restrict__ X
   // A missing row only fills its own (discarded) row of D, and k >= K meets the zero
   // rows of the packed operand.  Splits are multiples of 16 columns, so a step never
   // crosses into the next split.
-  const float *p0 = X + (mb + i < M ? mb + i : M - 1);
-  const float *p1 = X + (mb + 32 + i < M ? mb + 32 + i : M - 1);
-  for (int64_t k0 = kbeg; k0 < kend; k0 += 16) {
+  const int64_t row0 = mb + i < M ? mb + i : M - 1, row1 = mb + 32 + i < M ? mb + 32 + i : M - 1;
+  const float *p0 = X + row0;
+  const float *p1 = X + row1;
+  auto step = [&](int64_t k0, const float (&xc)[2][8]) {
     const int64_t ks = k0 >> 4;
     bf16x8 bh[NT], bl[NT];
     if constexpr (MODE == kGemmBf16x3) {
@@ -155,6 +168,33 @@ __global__ __launch_bounds__(256) void k_gemm_cm_f32(const float *__restrict__ X
         bl[t] = wlo[(ks * NT + t) * 64 + lane];
       }
     }
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const float *wrow = wf ? wf + k0 * RPF : nullptr;
+      mma_step<MODE>(xc[0], bh[t], bl[t], wrow, RPF, kg, 32 * t + i, acc[0][t]);
+      mma_step<MODE>(xc[1], bh[t], bl[t], wrow, RPF, kg, 32 * t + i, acc[1][t]);
+    }
+  };
+  int64_t k0 = kbeg;
+  if constexpr (BUF) {
+    // whole 16-k steps through a raw buffer over the block's k range (its byte size fits 32
+    // bits, checked by the launcher): the lane's row and k group in one offset register, the
+    // step's row in a scalar one, no 64-bit address per load
+    const __amdgpu_buffer_rsrc_t rx = raw_rsrc(X + kbeg * M, (uint32_t)((kend - kbeg) * M * 4));
+    const uint32_t vo0 = (uint32_t)((row0 + 8 * kg * M) * 4), vo1 = (uint32_t)((row1 + 8 * kg * M) * 4);
+    const uint32_t rowb = __builtin_amdgcn_readfirstlane((uint32_t)(M * 4));
+    for (; k0 + 16 <= kend; k0 += 16) {
+      const uint32_t sb = __builtin_amdgcn_readfirstlane((uint32_t)((k0 - kbeg) * M * 4));
+      float xc[2][8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        xc[0][j] = load_nt(rx, vo0, sb + j * rowb);
+        xc[1][j] = load_nt(rx, vo1, sb + j * rowb);
+      }
+      step(k0, xc);
+    }
+  }
+  for (; k0 < kend; k0 += 16) {
     float xc[2][8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
@@ -163,12 +203,7 @@ __global__ __launch_bounds__(256) void k_gemm_cm_f32(const float *__restrict__ X
       xc[0][j] = __builtin_nontemporal_load(p0 + M * kc);
       xc[1][j] = __builtin_nontemporal_load(p1 + M * kc);
     }
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      const float *wrow = wf ? wf + k0 * RPF : nullptr;
-      mma_step<MODE>(xc[0], bh[t], bl[t], wrow, RPF, kg, 32 * t + i, acc[0][t]);
-      mma_step<MODE>(xc[1], bh[t], bl[t], wrow, RPF, kg, 32 * t + i, acc[1][t]);
-    }
+    step(k0, xc);
   }
   float *o = out + (int64_t)blockIdx.y * M * RPF;
   float *st = stg[wave];
@@ -514,8 +549,13 @@ void gemm_cm_f32(hipStream_t s, GemmMode mode, const float *x32, int64_t M, int6
   kchunk = (kchunk + 15) / 16 * 16;
   dim3 grid((unsigned)((M + 255) / 256), (unsigned)splits);
   const bf16x8 *h = (const bf16x8 *)whi, *l = (const bf16x8 *)wlo;
-#define PTN_CM32(MODEV, NTV, CAPV) \
-  k_gemm_cm_f32<MODEV, NTV, CAPV><<<grid, 256, 0, s>>>(x32, M, K, kchunk, h, l, wf, out)
+  // raw-buffer loads when a block's k range of the tensor spans less than 4 GB
+  const bool buf = (kchunk + 16) * M * 4 < ((int64_t)1 << 32);
+#define PTN_CM32(MODEV, NTV, CAPV)                                                                 \
+  do {                                                                                            \
+    if (buf) k_gemm_cm_f32<MODEV, NTV, CAPV, true><<<grid, 256, 0, s>>>(x32, M, K, kchunk, h, l, wf, out); \
+    else k_gemm_cm_f32<MODEV, NTV, CAPV, false><<<grid, 256, 0, s>>>(x32, M, K, kchunk, h, l, wf, out); \
+  } while (0)
   if (mode == kGemmBf16x3) {
     if (RPF == 32) PTN_CM32(kGemmBf16x3, 1, true);
     else if (RPF == 64) PTN_CM32(kGemmBf16x3, 2, true);
