@@ -101,6 +101,28 @@ def test_mttkrp_full_size_closed_form(h):
         assert P.rel_err(D3[mode], ref) < 1e-4, mode
 
 
+def test_left_node_over_4gb_k_range_closed_form(h):
+    """1100 x 1000 x 1000 FP32 (4.4 GB): the left node's k range spans more than 4 GB, so the
+    strided contraction loads through 64-bit addresses instead of a raw buffer; the tree's
+    MTTKRPs of the exact rank-8 tensor still match the closed form M_n = B_n (o B_k^T A_k)."""
+    dims, Rt, R = [1100, 1000, 1000], 8, 32
+    say("left node over a 4 GB k range", dims)
+    h.generate_tensor(dims, Rt, seed=0, snr_db=None, storage=P.TENSOR_F32)
+    B = o.gen_factors(dims, Rt, seed=0)
+    A = o.gen_factors(dims, R, seed=1)
+    for ar, tol in ((P.ARITH_F32, 1e-5), (P.ARITH_BF16X3, 1e-4)):
+        D = h.mttkrp_dimtree(A, arith=ar)
+        for mode in range(3):
+            H = np.ones((Rt, R))
+            for k in range(3):
+                if k != mode:
+                    H *= B[k].T @ A[k]
+            err = P.rel_err(D[mode], B[mode] @ H)
+            say("arith", ar, "mode", mode, "rel err", err)
+            assert err < tol, (ar, mode, err)
+    h.set_tensor(np.ones((2, 2, 2), order="F"))          # release the 4.4 GB
+
+
 # --------------------------------------------------------- generator / cost --
 
 def test_generator_matches_oracle(h):
