@@ -44,8 +44,11 @@ def test_mttkrp_golden_exact(h, golden_dir, name):
 
 
 @pytest.mark.parametrize("dims,R", [([100, 100, 100], 10), ([37, 23, 41], 5), ([17, 65, 33], 33),
-                                    ([64, 16, 48], 64), ([9, 11, 13, 7], 17), ([3, 5, 4, 2, 3], 6)])
+                                    ([64, 16, 48], 64), ([9, 11, 13, 7], 17), ([3, 5, 4, 2, 3], 6),
+                                    ([20, 20, 30, 31], 40)])
 def test_mttkrp_vs_oracle(h, dims, R):
+    """Every mode and form against the oracle; [20, 20, 30, 31] gives the order-4 left node
+    three k splits (400 rows, k = 930) whose last ends in a partial 16-k step."""
     say("mttkrp", dims, R)
     rng = np.random.default_rng(sum(dims) + R)
     X = np.asfortranarray(rng.standard_normal(dims))
