@@ -36,6 +36,7 @@ class Options(ctypes.Structure):
         ("acceleration", _i), ("accel_coeff", _i), ("accel_fail", _i), ("normalization", _i),
         ("constraints", _i * MAX_ORDER), ("mttkrp_form", _i), ("cost_form", _i),
         ("threads", _i), ("max_inner", _i),
+        ("force_accept", ctypes.POINTER(ctypes.c_int)), ("force_n", _i),
     ]
 
 
@@ -47,7 +48,7 @@ class Status(ctypes.Structure):
 class SweepLog(ctypes.Structure):
     _fields_ = [
         ("f_value", _d), ("rel_cost", _d), ("f_accel", _d), ("accel_tried", _i), ("accel_accepted", _i),
-        ("nnls_iters", _i * MAX_ORDER), ("zero_col_fallback", _i * MAX_ORDER),
+        ("accel_overridden", _i), ("nnls_iters", _i * MAX_ORDER), ("zero_col_fallback", _i * MAX_ORDER),
         ("lipschitz", _d * MAX_ORDER), ("mu", _d * MAX_ORDER), ("t_start", _d), ("t_end", _d),
     ]
 
@@ -265,6 +266,18 @@ def inverse(V):
     return out
 
 
+def update_factor(constraint, V, M, A0, d1=1e-2, d2=1e-2, max_inner=50):
+    """update_factor (Cpd.hpp:189-228) on one factor: returns (A, G, nesterov steps, fallback)."""
+    V = np.asfortranarray(V, dtype=np.float64)
+    M = np.asfortranarray(M, dtype=np.float64)
+    A = np.array(A0, dtype=np.float64, order="F")
+    G = np.empty_like(V, order="F")
+    fb = _i()
+    it = lib().ptno_update_factor(_i(constraint), _i64(A.shape[0]), _i(A.shape[1]), _p(V), _p(M), _d(d1), _d(d2),
+                                  _i(max_inner), _p(A), _p(G), ctypes.byref(fb))
+    return A, G, it, fb.value
+
+
 def normalize(grams, factors, weight_factor=0):
     dims = [f.shape[0] for f in factors]
     R = factors[0].shape[1]
@@ -310,7 +323,7 @@ def _log_to_dict(rec, N):
     return {
         "f_value": rec.f_value, "rel_cost": rec.rel_cost, "f_accel": rec.f_accel,
         "accel_tried": rec.accel_tried, "accel_accepted": rec.accel_accepted,
-        "nnls_iters": list(rec.nnls_iters[:N]), "zero_col_fallback": list(rec.zero_col_fallback[:N]),
+        "accel_overridden": rec.accel_overridden, "nnls_iters": list(rec.nnls_iters[:N]), "zero_col_fallback": list(rec.zero_col_fallback[:N]),
         "lipschitz": list(rec.lipschitz[:N]), "mu": list(rec.mu[:N]),
         "t_start": rec.t_start, "t_end": rec.t_end,
     }
@@ -322,7 +335,13 @@ def cpd(X, R, init, **kw):
     (factors, status dict, per-sweep log)."""
     dims = X.shape
     N = len(dims)
+    force = kw.pop("force_accept", None)
     opt = make_options(**kw)
+    if force is not None:
+        # forced line-search outcomes per sweep (1 accept, 0 reject, -1 the oracle's own test)
+        fbuf = (ctypes.c_int * max(len(force), 1))(*[int(v) for v in force])
+        opt.force_accept = ctypes.cast(fbuf, ctypes.POINTER(ctypes.c_int))
+        opt.force_n = len(force)
     out = np.empty(sum(dims) * R, dtype=np.float64)
     st = Status()
     cap = max(int(opt.max_iter), 1)

@@ -995,11 +995,12 @@ static void mttkrp_form(const ptno_options *opt, int N, const int64_t *dims, int
     ptno_mttkrp_naive(N, dims, R, X, packed, mode, M);
 }
 
-/* update_factor, Cpd.hpp:189-228 (dimension tree: CpdDimTree.hpp:199-239). */
-static void update_factor(const ptno_options *opt, int idx, int64_t I, int R, const double *V,
-                          const double *M, double *A, double *G, ptno_sweep_log *lg)
+/* The constraint switch of update_factor (Cpd.hpp:194-225) and the Gram of Cpd.hpp:227. */
+static int update_one(int constraint, int64_t I, int R, const double *V, const double *M, double d1, double d2,
+                      int max_inner, double *A, double *G, int *fb_out, double *L_out, double *mu_out)
 {
-  switch (opt->constraints[idx]) {
+  int it = 0;
+  switch (constraint) {
     case PTNO_UNCONSTRAINED: {                         /* A = M V^{-1} */
       double *Vi = (double *)malloc(sizeof(double) * R * R);
       ptno_inverse(R, V, Vi);
@@ -1015,8 +1016,7 @@ static void update_factor(const ptno_options *opt, int idx, int64_t I, int R, co
     case PTNO_NONNEGATIVITY: {
       double *old = (double *)malloc(sizeof(double) * I * R);
       memcpy(old, A, sizeof(double) * I * R);
-      double L = 0, mu = 0;
-      int it = ptno_nesterov_mnls(I, R, V, M, opt->delta1, opt->delta2, opt->max_inner, A, &L, &mu);
+      it = ptno_nesterov_mnls(I, R, V, M, d1, d2, max_inner, A, L_out, mu_out);
       /* zero-column fallback, Cpd.hpp:206-207 */
       double mincol = INFINITY;
       for (int c = 0; c < R; ++c) {
@@ -1029,7 +1029,7 @@ static void update_factor(const ptno_options *opt, int idx, int64_t I, int R, co
         fb = 1;
         for (int64_t e = 0; e < I * R; ++e) A[e] = 0.9 * A[e] + 0.1 * old[e];
       }
-      if (lg) { lg->nnls_iters[idx] = it; lg->zero_col_fallback[idx] = fb; lg->lipschitz[idx] = L; lg->mu[idx] = mu; }
+      *fb_out = fb;
       free(old);
       break;
     }
@@ -1058,6 +1058,30 @@ static void update_factor(const ptno_options *opt, int idx, int64_t I, int R, co
     default: break;                                     /* sparsity / constant: unchanged */
   }
   ptno_gram(I, R, A, G);
+  return it;
+}
+
+/* update_factor, Cpd.hpp:189-228 (dimension tree: CpdDimTree.hpp:199-239). */
+static void update_factor(const ptno_options *opt, int idx, int64_t I, int R, const double *V,
+                          const double *M, double *A, double *G, ptno_sweep_log *lg)
+{
+  int fb = 0;
+  double L = 0, mu = 0;
+  int it = update_one(opt->constraints[idx], I, R, V, M, opt->delta1, opt->delta2, opt->max_inner, A, G, &fb,
+                      &L, &mu);
+  if (lg && opt->constraints[idx] == PTNO_NONNEGATIVITY) {
+    lg->nnls_iters[idx] = it; lg->zero_col_fallback[idx] = fb; lg->lipschitz[idx] = L; lg->mu[idx] = mu;
+  }
+}
+
+int ptno_update_factor(int constraint, int64_t I, int R, const double *V, const double *M, double d1,
+                       double d2, int max_inner, double *A, double *G, int *fallback)
+{
+  double L = 0, mu = 0;
+  int fb = 0;
+  int it = update_one(constraint, I, R, V, M, d1, d2, max_inner, A, G, &fb, &L, &mu);
+  if (fallback) *fallback = fb;
+  return it;
 }
 
 int ptno_cpd(int N, const int64_t *dims, int R, const double *X, const double *init_packed,
@@ -1145,7 +1169,13 @@ int ptno_cpd(int N, const int64_t *dims, int R, const double *X, const double *i
         double fa = sqrt(st->frob_sq + vg - 2.0 * ma);
         lrec.accel_tried = 1;
         lrec.f_accel = fa;
-        if (st->f_value > fa) {
+        int accept = st->f_value > fa;
+        if (opt->force_accept && st->ao_iter <= opt->force_n && opt->force_accept[st->ao_iter - 1] >= 0) {
+          int forced = opt->force_accept[st->ao_iter - 1] != 0;
+          lrec.accel_overridden = forced != accept;
+          accept = forced;
+        }
+        if (accept) {
           memcpy(A, acc, sizeof(double) * fsz);
           memcpy(grams, agr, sizeof(double) * N * R * R);
           st->f_value = fa;

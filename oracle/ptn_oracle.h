@@ -50,6 +50,14 @@ typedef struct {
   int    cost_form;
   int    threads;             /* OpenMP threads for the blocked form, 0 = runtime default */
   int    max_inner;           /* Nesterov cap, NesterovMNLS.hpp:187 (50) */
+  /* Forced line-search outcomes (test hook): when force_accept is not NULL and sweep k
+     (1-based) has k <= force_n and force_accept[k - 1] >= 0, the step of sweep k is
+     accepted iff force_accept[k - 1] != 0, instead of by `f_value > f_accel`
+     (Cpd.hpp:263); the counters of Cpd.hpp:270-277 follow the forced outcome.  A parity
+     test passes the device's outcomes so that both sides stay on one path after a
+     near-tie, and checks separately that every overridden sweep is such a tie. */
+  const int *force_accept;
+  int    force_n;
 } ptno_options;
 
 typedef struct {
@@ -68,6 +76,7 @@ typedef struct {
   double f_accel;            /* accelerated cost, 0 when not tried */
   int    accel_tried;
   int    accel_accepted;
+  int    accel_overridden;   /* 1 when a forced outcome differs from the oracle's own test */
   int    nnls_iters[PTNO_MAX_ORDER];
   int    zero_col_fallback[PTNO_MAX_ORDER];
   double lipschitz[PTNO_MAX_ORDER];   /* L and mu of V_n before GLambda */
@@ -111,6 +120,11 @@ double ptno_update_alpha(double alpha, double q);                               
 int    ptno_nesterov_mnls(int64_t I, int R, const double *Z, const double *W, double d1, double d2,
                           int max_inner, double *A, double *L_out, double *mu_out);    /* NesterovMNLS.hpp:181-234 */
 int    ptno_inverse(int R, const double *V, double *Vinv);                               /* Eigen .inverse() (PartialPivLU) */
+/* update_factor, Cpd.hpp:189-228, on one factor: constraint (PTNO_*), V (R x R), M and A
+   (I x R col-major; A in/out), G = A^T A out.  Returns the Nesterov step count (0 for the
+   other branches); *fallback = 1 when the zero-column blend of Cpd.hpp:206-207 ran. */
+int    ptno_update_factor(int constraint, int64_t I, int R, const double *V, const double *M, double d1,
+                          double d2, int max_inner, double *A, double *G, int *fallback);
 
 /* Normalize (Normalize.hpp:92-174), grams/factors packed. */
 void ptno_normalize(int N, const int64_t *dims, int R, int weight_factor, double *grams_packed, double *packed);

@@ -439,15 +439,6 @@ class Handle:
         return ms.value, b.value
 
 
-def gpu_available():
-    """True when a HIP device is visible (counts devices without initialising one)."""
-    try:
-        import torch
-        return torch.cuda.device_count() > 0
-    except Exception:
-        return False
-
-
 def rel_err(a, b):
     a = np.asarray(a)
     b = np.asarray(b)

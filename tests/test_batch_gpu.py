@@ -4,8 +4,10 @@ The batch has no reference counterpart (the reference runs one cpd() per tensor)
 pinned by running the oracle's cpd() (standard MTTKRP, Gram-form cost) on every problem
 with the same FP32-rounded tensor and the same initial factors.  The MTTKRP summation
 order differs from the oracle's, so costs and factors agree to the tolerances written
-below, while every discrete decision (Nesterov step counts, accepted line searches,
-threshold stops) must agree exactly."""
+below (1e-12 relative for the FP64 batch), while every discrete decision (Nesterov step
+counts, accepted line searches, threshold stops) must agree exactly.  A line search whose
+two costs tie to rounding may go either way: the oracle follows the device's outcomes and
+every sweep where they differ from its own test must be such a tie."""
 import numpy as np
 import pytest
 
@@ -50,31 +52,44 @@ def _tensors(probs, snr_db=25.0, seed0=0):
     return Xs
 
 
-def _check_problem(p, Fg, stg, logg, Fo, sto, logo, ftol, atol):
-    """Sweep by sweep: equal step counts and line-search outcomes, costs within ftol.  A
-    line search whose two costs tie to 1e-9 in the oracle may go either way (the costs
-    carry ~1e-13 of summation-order difference): the comparison ends at such a sweep,
-    and the problem must still run the same number of sweeps.  Without a tie the final
-    factors agree within atol."""
+TOL = 1e-12       # FP64 batch: factors against the oracle, relative
+COST_TOL = 2e-13  # |f_device^2 - f_oracle^2| / ||X||^2: the Gram-form cost is a difference of
+                  # three terms of size ||X||^2, each a sum of up to 2*10^6 products taken in
+                  # another order on each side (measured <= 6.5e-14)
+TIE = 1e-12       # |f - f_accel| / f below which a line-search outcome rests on rounding
+
+
+def oracle_problem(X, R, init, logg, **kw):
+    """The oracle's cpd() of one problem on the device's line-search outcomes; every sweep
+    where they differ from the oracle's own test must be a tie to TIE."""
+    force = [l["accel_accepted"] if l["accel_tried"] else -1 for l in logg]
+    Fo, sto, logo = o.cpd(X, R, init, force_accept=force, cost_form=o.COST_GRAM, **kw)
+    for k, l in enumerate(logo):
+        if l["accel_overridden"]:
+            gap = abs(l["f_value"] - l["f_accel"]) / l["f_value"]
+            say("line-search tie at sweep %d: oracle |f - f_accel| / f = %.2e" % (k + 1, gap))
+            assert gap <= TIE, (k, l["f_value"], l["f_accel"])
+    return Fo, sto, logo
+
+
+def _check_problem(p, Fg, stg, logg, Fo, sto, logo, tol=TOL):
+    """Sweep by sweep: equal step counts and line-search outcomes, costs within COST_TOL of
+    ||X||^2; final factors within tol, status and counters.  Returns (factor gap, cost gap)."""
     assert stg["ao_iter"] == sto["ao_iter"], p
-    tie = None
+    frob = sto["frob_sq"]
+    cgap = 0.0
     for k, (a, b) in enumerate(zip(logg, logo)):
-        assert a["f_value"] == pytest.approx(b["f_value"], rel=ftol), (p, k)
+        cgap = max(cgap, abs(a["f_value"] ** 2 - b["f_value"] ** 2) / frob)
         assert a["nnls_iters"] == b["nnls_iters"][:3], (p, k)
         assert a["accel_tried"] == b["accel_tried"], (p, k)
+        assert a["accel_accepted"] == b["accel_accepted"], (p, k)
         if b["accel_tried"]:
-            assert a["f_accel"] == pytest.approx(b["f_accel"], rel=ftol), (p, k)
-        if a["accel_accepted"] != b["accel_accepted"]:
-            assert abs(b["f_value"] - b["f_accel"]) <= 1e-9 * b["f_value"], (p, k, b["f_value"], b["f_accel"])
-            tie = k
-            break
-    if tie is not None:
-        say("problem", p, "line-search tie at sweep", tie + 1, "- compared up to there")
-        return
+            cgap = max(cgap, abs(a["f_accel"] ** 2 - b["f_accel"] ** 2) / frob)
+    assert cgap < COST_TOL, (p, cgap)
     assert (stg["accel_coeff"], stg["accel_fail"]) == (sto["accel_coeff"], sto["accel_fail"]), p
-    assert stg["f_value"] == pytest.approx(sto["f_value"], rel=ftol), p
-    for a, b in zip(Fg, Fo):
-        assert P.rel_err(a, b) < atol, (p, P.rel_err(a, b))
+    err = max(P.rel_err(a, b) for a, b in zip(Fg, Fo))
+    assert err < tol, (p, err)
+    return err, cgap
 
 
 def test_batch_parity_vs_oracle(h):
@@ -86,10 +101,12 @@ def test_batch_parity_vs_oracle(h):
     kw = dict(max_iter=12, threshold_error=0.0)
     Fg, stg, logg, rep = h.cpd_batch(init, block=256, grid=40, **kw)
     say("report", rep)
+    worst = [0.0, 0.0]
     for p, ((dims, R), X) in enumerate(zip(probs, Xs)):
-        Fo, sto, logo = o.cpd(X, R, init[p], cost_form=o.COST_GRAM, **kw)
-        _check_problem(p, Fg[p], stg[p], logg[p], Fo, sto, logo, 1e-10, 1e-9)
+        Fo, sto, logo = oracle_problem(X, R, init[p], logg[p], **kw)
+        worst = [max(a, b) for a, b in zip(worst, _check_problem(p, Fg[p], stg[p], logg[p], Fo, sto, logo))]
         assert stg[p]["frob_sq"] == pytest.approx(sto["frob_sq"], rel=1e-13)
+    say("over the batch vs oracle: max factor rel diff %.2e, max |f^2 - f_o^2| / ||X||^2 %.2e" % tuple(worst))
 
 
 def test_batch_fp32_matrix_cores_vs_fp64(h):
@@ -160,11 +177,11 @@ def test_batch_threshold_stop(h):
     kw = dict(max_iter=120, threshold_error=0.05)
     Fg, stg, logg, _ = h.cpd_batch(init, block=256, grid=2, **kw)
     for p, ((dims, R), X) in enumerate(zip(probs, Xs)):
-        Fo, sto, logo = o.cpd(X, R, init[p], cost_form=o.COST_GRAM, **kw)
+        Fo, sto, logo = oracle_problem(X, R, init[p], logg[p], **kw)
         say("problem", p, "stops at", stg[p]["ao_iter"], "oracle", sto["ao_iter"])
         assert 1 < sto["ao_iter"] < 120
         assert logg[p][-1]["stopped"] == 1 and logg[p][-1]["accel_tried"] == 0
-        _check_problem(p, Fg[p], stg[p], logg[p], Fo, sto, logo, 1e-10, 1e-9)
+        _check_problem(p, Fg[p], stg[p], logg[p], Fo, sto, logo)
 
 
 def test_batch_no_accel_no_norm(h):
@@ -175,8 +192,8 @@ def test_batch_no_accel_no_norm(h):
     kw = dict(max_iter=6, threshold_error=0.0, acceleration=0, normalization=0)
     Fg, stg, logg, _ = h.cpd_batch(init, block=512, grid=4, **kw)
     for p, ((dims, R), X) in enumerate(zip(probs, Xs)):
-        Fo, sto, logo = o.cpd(X, R, init[p], cost_form=o.COST_GRAM, **kw)
-        _check_problem(p, Fg[p], stg[p], logg[p], Fo, sto, logo, 1e-10, 1e-9)
+        Fo, sto, logo = oracle_problem(X, R, init[p], logg[p], **kw)
+        _check_problem(p, Fg[p], stg[p], logg[p], Fo, sto, logo)
 
 
 def test_batch_generator_and_init_match_oracle(h):

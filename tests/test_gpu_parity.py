@@ -1,7 +1,12 @@
 """GPU parity: the HIP path through the C ABI against the FP64 CPU oracle on the
 same inputs.  Integer-valued golden data must match exactly; FP64 runs within
-the tolerances written in each test.  Full-size (800^3) checks use properties:
-the closed form of an exact low-rank MTTKRP."""
+the tolerances written in each test (1e-12 relative for the plain FP64 path's
+factors and costs).  ALS runs compare every discrete decision; a line-search test
+whose two costs tie to rounding may go either way, so the oracle is run on the
+device's line-search outcomes and every sweep where they differ from its own test
+must be such a tie -- the final factors are compared in every case.  Full-size
+checks use the closed form of an exact low-rank MTTKRP (800^3, 200^4) and a
+three-sweep 800^3 ALS against the oracle."""
 import os
 import sys
 
@@ -178,66 +183,119 @@ def test_nesterov_matches_oracle(h, I, R, d):
     assert np.array_equal(Ag, Ao)                          # row arithmetic mirrors the oracle
 
 
-def test_normalize_matches_oracle(h):
-    rng = np.random.default_rng(3)
+@pytest.mark.parametrize("wf", [0, 1, 2, 3])
+def test_normalize_matches_oracle(h, wf):
+    """Normalize with the weights loaded on factor wf (Normalize.hpp:92-174): bitwise the
+    oracle's, including the zero-norm branch, for every weight factor of orders 3 and 4."""
+    rng = np.random.default_rng(3 + wf)
     for N in (3, 4):
+        if wf >= N:
+            continue
         dims = [7, 6, 5, 4][:N]
         A = [rng.random((I, 5)) for I in dims]
         A[-1][:, 2] = 0.0                                  # zero-norm branch on one column
+        A[N - 2][:, 4] = 0.0                               # ... the one wf = N - 1 reads
         G = [a.T @ a for a in A]
-        say("normalize N=%d" % N)
-        Gg, Ag = h.normalize(G, A, 0)
-        Go, Ao = o.normalize(G, A, 0)
-        for a, b in zip(Ag, Ao):
-            assert np.array_equal(a, b)
+        say("normalize N=%d wf=%d" % (N, wf))
+        Gg, Ag = h.normalize(G, A, wf)
+        Go, Ao = o.normalize(G, A, wf)
+        for a, b in zip(Ag, Ao):           # a zero norm the test does not read divides by 0: NaN on both
+            assert np.array_equal(a, b, equal_nan=True)
         for a, b in zip(Gg, Go):
-            assert np.array_equal(a, b)
+            assert np.array_equal(a, b, equal_nan=True)
 
 
-def test_order4_gemm_node_full_size_errors(h):
-    """200^4 FP32 (6.4 GB), R = 64, Exponential(1) factors: the tree's MTTKRPs from the
-    FP32-MFMA and bf16x3 nodes against the FP64-MFMA path on the FP64 copy (itself
-    pinned to the oracle at small sizes).  Tolerances: FP32 accumulation over k =
-    40000 in split chunks."""
-    dims, R = [200, 200, 200, 200], 64
-    say("order-4 node 200^4 R=64")
-    h.generate_tensor(dims, R, seed=2, snr_db=20.0, dist=1, storage=P.TENSOR_BOTH)
+def test_order4_gemm_node_full_size_closed_form(h):
+    """200^4 (6.4 GB FP32, 12.8 GB FP64), noiseless and of exact rank 64 with Exponential(1)
+    factors: every MTTKRP of the tree -- the order-4 GEMM node in FP32 MFMA and bf16x3, and
+    the FP64 path on the FP64 copy -- against the closed form M_n = B_n (o_{k!=n} B_k^T A_k).
+    Tolerances: FP64 1e-12; FP32 rounding of X plus FP32 accumulation over k = 40000 in
+    split chunks 1e-5 (FP32 MFMA) and 1e-4 (bf16x3)."""
+    dims, Rt, R = [200, 200, 200, 200], 64, 64
+    say("order-4 node 200^4 R=64 closed form")
+    h.generate_tensor(dims, Rt, seed=2, snr_db=None, dist=1, storage=P.TENSOR_BOTH)
+    B = o.gen_factors(dims, Rt, seed=2, dist=1)
     A = h.generate_factors(dims, R, seed=1)
-    M64 = h.mttkrp_dimtree(A, arith=P.ARITH_F64)
-    for ar, tol in ((P.ARITH_F32, 1e-5), (P.ARITH_BF16X3, 1e-4)):
+    refs = []
+    for mode in range(4):
+        H = np.ones((Rt, R))
+        for k in range(4):
+            if k != mode:
+                H *= B[k].T @ A[k]
+        refs.append(B[mode] @ H)
+    for ar, tol in ((P.ARITH_F64, 1e-12), (P.ARITH_F32, 1e-5), (P.ARITH_BF16X3, 1e-4)):
         M = h.mttkrp_dimtree(A, arith=ar)
-        errs = [P.rel_err(M[k], M64[k]) for k in range(4)]
-        say("arith", ar, "rel errs", errs)
-        assert max(errs) < tol
+        errs = [P.rel_err(M[k], refs[k]) for k in range(4)]
+        say("arith", ar, "rel errs vs closed form", ["%.2e" % e for e in errs])
+        assert max(errs) < tol, (ar, errs)
     h.set_tensor(np.ones((2, 2, 2), order="F"))          # release the 19 GB
 
 
 # --------------------------------------------------------------------- ALS --
 
-def _compare_runs(Fg, stg, logg, Fo, sto, logo, tol, final=True):
+TOL = 1e-12       # plain FP64 path: factors, relative
+COST_TOL = 2e-13  # |f_device^2 - f_oracle^2| / ||X||^2 (see _compare_runs)
+TIE = 1e-12       # |f - f_accel| / f below which a line-search outcome rests on rounding
+
+
+def oracle_on_device_path(X, R, init, logg, tie=TIE, **kw):
+    """The oracle's cpd() following the device's line-search outcomes; every sweep where
+    they differ from the oracle's own test f > f_accel must be a tie to `tie`."""
+    force = [l["accel_accepted"] if l["accel_tried"] else -1 for l in logg]
+    Fo, sto, logo = o.cpd(X, R, init, force_accept=force, **kw)
+    for k, l in enumerate(logo):
+        if l["accel_overridden"]:
+            gap = abs(l["f_value"] - l["f_accel"]) / l["f_value"]
+            say("line-search tie at sweep %d: oracle |f - f_accel| / f = %.2e" % (k + 1, gap))
+            assert gap <= tie, (k, l["f_value"], l["f_accel"])
+    return Fo, sto, logo
+
+
+def _compare_runs(Fg, stg, logg, Fo, sto, logo, tol=TOL, cost_tol=COST_TOL):
+    """Every discrete decision equal; factors within tol.  Costs are compared on the scale
+    of ||X||^2: the Gram-form cost f^2 = ||X||^2 - 2<M, A> + <V, G> is a difference of
+    three terms of that size, each a sum of up to 10^6 products taken in another order on
+    each side (the oracle sums ||X||^2 sequentially in chunks of 65 536, ~sqrt(65536) eps =
+    3e-14 of rounding), so f^2 agrees to COST_TOL of ||X||^2 (measured <= 2.2e-14 here,
+    6.5e-14 in the batch tests), which near the noise floor is ~1e-12 of f itself."""
     assert stg["ao_iter"] == sto["ao_iter"]
     assert [l["nnls_iters"] for l in logg] == [l["nnls_iters"] for l in logo]
+    assert [l["accel_tried"] for l in logg] == [l["accel_tried"] for l in logo]
     assert [l["accel_accepted"] for l in logg] == [l["accel_accepted"] for l in logo]
     assert [l["zero_col_fallback"] for l in logg] == [l["zero_col_fallback"] for l in logo]
-    for a, b in zip(logg, logo):
-        assert a["f_value"] == pytest.approx(b["f_value"], rel=tol)
-    if not final:
-        return
+    frob = sto["frob_sq"]
+    fgap = max(abs(a["f_value"] ** 2 - b["f_value"] ** 2) for a, b in zip(logg, logo)) / frob
+    agap = max([abs(a["f_accel"] ** 2 - b["f_accel"] ** 2) for a, b in zip(logg, logo) if b["accel_tried"]]
+               or [0.0]) / frob
+    ferr = max(P.rel_err(a, b) for a, b in zip(Fg, Fo))
+    say("vs oracle: final factors %.2e (tolerance %.1e); max |f^2 - f_o^2| / ||X||^2 %.2e, accelerated %.2e "
+        "(tolerance %.1e)" % (ferr, tol, fgap, agap, cost_tol))
+    assert fgap < cost_tol and agap < cost_tol
     assert (stg["accel_coeff"], stg["accel_fail"]) == (sto["accel_coeff"], sto["accel_fail"])
-    assert stg["f_value"] == pytest.approx(sto["f_value"], rel=tol)
-    for a, b in zip(Fg, Fo):
-        assert P.rel_err(a, b) < tol
+    assert ferr < tol
 
 
-def _first_accel_tie(logg, logo):
-    """The first sweep whose line-search outcome differs, if the oracle's two costs tie
-    there to 1e-9 (costs carry ~1e-14 of summation-order difference, so such a test may go
-    either way); None when the outcomes agree throughout."""
-    for k, (a, b) in enumerate(zip(logg, logo)):
-        if a["accel_accepted"] != b["accel_accepted"]:
-            assert abs(b["f_value"] - b["f_accel"]) <= 1e-9 * b["f_value"], (k, b["f_value"], b["f_accel"])
-            return k
-    return None
+def _als_parity(h, X, R, init, oform=o.MTTKRP_STANDARD, **kw):
+    """One device run through the C ABI against the oracle's run on its path.  The factor
+    (cost) tolerance is 1e-12 (COST_TOL), or four times the spread between the oracle's own
+    two summation orders (per-mode and blocked MTTKRP, same path) where the problem
+    amplifies rounding beyond that: the orthogonality branch's (M^T M)^-1/2 and the
+    unconstrained M V^-1 reach a 2.5e-11 spread after ten sweeps in the oracle alone."""
+    Fg, stg, logg = h.cpd(R, init, **kw)
+    okw = {k: v for k, v in kw.items() if k != "mttkrp_form"}
+    Fo, sto, logo = oracle_on_device_path(X, R, init, logg, mttkrp_form=oform, cost_form=o.COST_GRAM, **okw)
+    other = o.MTTKRP_BLOCKED if oform != o.MTTKRP_BLOCKED else o.MTTKRP_STANDARD
+    Fs, _, logs = oracle_on_device_path(X, R, init, logg, mttkrp_form=other, cost_form=o.COST_GRAM, **okw)
+    spread = max(P.rel_err(a, b) for a, b in zip(Fs, Fo))
+    cspread = max(abs(a["f_value"] ** 2 - b["f_value"] ** 2) for a, b in zip(logs, logo)) / sto["frob_sq"]
+    say("oracle's own spread between summation orders: factors %.2e, |f^2 - f^2| / ||X||^2 %.2e"
+        % (spread, cspread))
+    _compare_runs(Fg, stg, logg, Fo, sto, logo, max(TOL, 4.0 * spread), max(COST_TOL, 4.0 * cspread))
+    return Fg, stg, logg
+
+
+def _oform(form):
+    return o.MTTKRP_DIMTREE if form == P.MTTKRP_DIMTREE else o.MTTKRP_STANDARD
 
 
 @pytest.mark.parametrize("form", [P.MTTKRP_STANDARD, P.MTTKRP_DIMTREE])
@@ -248,12 +306,9 @@ def test_als_c1_parity(h, form):
     X, _ = o.make_tensor(dims, R, seed=0, snr_db=30.0)
     init = o.gen_factors(dims, R, seed=1)
     h.set_tensor(X)
-    kw = dict(max_iter=50, threshold_error=0.0)
-    Fg, stg, logg = h.cpd(R, init, mttkrp_form=form, **kw)
-    Fo, sto, logo = o.cpd(X, R, init, mttkrp_form=(o.MTTKRP_DIMTREE if form == P.MTTKRP_DIMTREE else
-                                                     o.MTTKRP_STANDARD), cost_form=o.COST_GRAM, **kw)
-    say("rel", stg["rel_cost"], sto["rel_cost"], "sweeps/s", stg["ao_iter"] / stg["sweep_seconds"])
-    _compare_runs(Fg, stg, logg, Fo, sto, logo, 1e-10)
+    Fg, stg, logg = _als_parity(h, X, R, init, mttkrp_form=form, oform=_oform(form), max_iter=50,
+                                threshold_error=0.0)
+    say("rel", stg["rel_cost"], "sweeps/s", stg["ao_iter"] / stg["sweep_seconds"])
 
 
 @pytest.mark.parametrize("form", [P.MTTKRP_STANDARD, P.MTTKRP_DIMTREE])
@@ -264,14 +319,11 @@ def test_als_threshold_stop(h, form):
     X, _ = o.make_tensor(dims, R, seed=8, snr_db=30.0)
     init = o.gen_factors(dims, R, seed=9)
     h.set_tensor(X)
-    kw = dict(max_iter=200, threshold_error=0.05)
-    Fg, stg, logg = h.cpd(R, init, mttkrp_form=form, **kw)
-    Fo, sto, logo = o.cpd(X, R, init, mttkrp_form=(o.MTTKRP_DIMTREE if form == P.MTTKRP_DIMTREE else
-                                                     o.MTTKRP_STANDARD), cost_form=o.COST_GRAM, **kw)
-    say("threshold stop at", stg["ao_iter"], "oracle", sto["ao_iter"])
-    assert 1 < sto["ao_iter"] < 200
+    Fg, stg, logg = _als_parity(h, X, R, init, mttkrp_form=form, oform=_oform(form), max_iter=200,
+                                threshold_error=0.05)
+    say("threshold stop at", stg["ao_iter"])
+    assert 1 < stg["ao_iter"] < 200
     assert logg[stg["ao_iter"] - 1]["accel_tried"] == 0
-    _compare_runs(Fg, stg, logg, Fo, sto, logo, 1e-10)
 
 
 def test_als_c1_no_accel_no_norm(h):
@@ -280,10 +332,7 @@ def test_als_c1_no_accel_no_norm(h):
     X, _ = o.make_tensor(dims, R, seed=0, snr_db=30.0)
     init = o.gen_factors(dims, R, seed=1)
     h.set_tensor(X)
-    kw = dict(max_iter=50, threshold_error=0.0, acceleration=0, normalization=0)
-    Fg, stg, logg = h.cpd(R, init, **kw)
-    Fo, sto, logo = o.cpd(X, R, init, cost_form=o.COST_GRAM, **kw)
-    _compare_runs(Fg, stg, logg, Fo, sto, logo, 1e-10)
+    _als_parity(h, X, R, init, max_iter=50, threshold_error=0.0, acceleration=0, normalization=0)
 
 
 @pytest.mark.parametrize("dims,R", [([12, 9, 10, 8], 4), ([30, 31, 32], 33)])
@@ -292,11 +341,41 @@ def test_als_small_parity(h, dims, R):
     X, _ = o.make_tensor(dims, min(R, 6), seed=2, snr_db=25.0)
     init = o.gen_factors(dims, R, seed=3)
     h.set_tensor(X)
-    kw = dict(max_iter=15, threshold_error=0.0)
-    for form, oform in ((P.MTTKRP_STANDARD, o.MTTKRP_STANDARD), (P.MTTKRP_DIMTREE, o.MTTKRP_DIMTREE)):
+    for form in (P.MTTKRP_STANDARD, P.MTTKRP_DIMTREE):
+        _als_parity(h, X, R, init, mttkrp_form=form, oform=_oform(form), max_iter=15, threshold_error=0.0)
+
+
+def test_als_c2_full_size_three_sweeps(h):
+    """Config 2 at full size (800^3, true rank 32, SNR 20 dB, R = 32, init seed 1): three
+    sweeps (the line search runs at sweep 2) of the plain FP64 path, per-mode and tree, against
+    the oracle (OpenMP blocked MTTKRP, Gram-form cost) at 1e-12; the FP32-stream forms (tree,
+    FP32 MFMA and bf16x3 with the fused line-search pass) end within 1e-6 of the oracle's
+    relative error.  The device generates the tensor (bitwise the oracle's generator, tested
+    above); its ||X||^2 is checked against the oracle's tensor."""
+    dims, R, sweeps = [800, 800, 800], 32, 3
+    say("C2 full size: oracle tensor")
+    X, _ = o.make_tensor(dims, R, seed=0, snr_db=20.0)
+    init = o.gen_factors(dims, R, seed=1)
+    say("C2 full size: device tensor")
+    h.generate_tensor(dims, R, seed=0, snr_db=20.0, storage=P.TENSOR_BOTH)
+    assert h.norm_sq() == pytest.approx(float(np.sum(X * X)), rel=1e-12)
+    kw = dict(max_iter=sweeps, threshold_error=0.0)
+    say("C2 full size: oracle run")
+    Fo, sto, logo = o.cpd(X, R, init, mttkrp_form=o.MTTKRP_BLOCKED, cost_form=o.COST_GRAM, threads=0, **kw)
+    assert logo[1]["accel_tried"] == 1
+    for form in (P.MTTKRP_STANDARD, P.MTTKRP_DIMTREE):
+        say("C2 full size: FP64 form", form)
         Fg, stg, logg = h.cpd(R, init, mttkrp_form=form, **kw)
-        Fo, sto, logo = o.cpd(X, R, init, mttkrp_form=oform, cost_form=o.COST_GRAM, **kw)
-        _compare_runs(Fg, stg, logg, Fo, sto, logo, 1e-10)
+        assert [l["accel_accepted"] for l in logg] == [l["accel_accepted"] for l in logo]
+        _compare_runs(Fg, stg, logg, Fo, sto, logo)
+    for ar in (P.ARITH_F32, P.ARITH_BF16X3):
+        Fc, stc, _ = h.cpd(R, init, mttkrp_form=P.MTTKRP_DIMTREE, arith=ar, fuse_accel=1, **kw)
+        gap = abs(stc["rel_cost"] - sto["rel_cost"])
+        ferr = max(P.rel_err(a, b) for a, b in zip(Fc, Fo))
+        say("C2 full size: arith", ar, "rel error %.10f vs oracle %.10f (gap %.2e), factors %.2e"
+            % (stc["rel_cost"], sto["rel_cost"], gap, ferr))
+        assert gap < 1e-6
+    h.set_tensor(np.ones((2, 2, 2), order="F"))          # release the 6 GB
 
 
 @pytest.mark.parametrize("arith", [P.ARITH_F32, P.ARITH_BF16X3])
@@ -315,11 +394,9 @@ def test_als_fp32_stream_and_fused_line_search(h, arith, dims, R):
     kw = dict(max_iter=25, threshold_error=0.0, mttkrp_form=P.MTTKRP_DIMTREE, arith=arith)
     F0, st0, lg0 = h.cpd(R, init, fuse_accel=0, **kw)
     F1, st1, lg1 = h.cpd(R, init, fuse_accel=1, **kw)
-    Fo, sto, logo = o.cpd(X, R, init, mttkrp_form=o.MTTKRP_DIMTREE, cost_form=o.COST_GRAM, max_iter=25,
-                          threshold_error=0.0)
     acc0 = [l["accel_accepted"] for l in lg0]
     acc1 = [l["accel_accepted"] for l in lg1]
-    say("accept unfused", acc0, "fused", acc1, "oracle", [l["accel_accepted"] for l in logo])
+    say("accept unfused", acc0, "fused", acc1)
     tried = [i for i, l in enumerate(lg0) if l["accel_tried"]]
     assert tried and [i for i, l in enumerate(lg1) if l["accel_tried"]] == tried
     # first line search: same state in both runs, the fit term <X, [[Ahat]]> summed in
@@ -333,18 +410,20 @@ def test_als_fp32_stream_and_fused_line_search(h, arith, dims, R):
         assert [l["nnls_iters"] for l in lg0] == [l["nnls_iters"] for l in lg1]
         for a, b in zip(F0, F1):
             assert np.array_equal(a, b)
-    # against the FP64 oracle: FP32 errors scale with ||X||^2, so squared residuals are
-    # compared on that scale.  The per-sweep cost tracks the oracle's until the first
-    # acceleration decision that a near tie sends the other way; afterwards the runs
-    # follow different (equally valid) ALS paths and only the final fit is compared.
-    acco = [l["accel_accepted"] for l in logo]
-    rel_o = o.cost_explicit(X, Fo) / np.sqrt(frob)
-    for F, lg, acc in ((F0, lg0, acc0), (F1, lg1, acc1)):
-        dvg = next((i for i, (a, b) in enumerate(zip(acc, acco)) if a != b), len(acco))
-        for i in range(dvg):
-            assert abs(lg[i]["f_value"] ** 2 - logo[i]["f_value"] ** 2) < 1e-5 * frob, i
+    # against the FP64 oracle on each run's own line-search path: FP32 errors scale with
+    # ||X||^2 (the fit term <M, A> carries ~1e-7 of it), so squared residuals are compared on
+    # that scale, and a line-search outcome is a tie when the oracle's two costs are within
+    # the ~1e-4 relative error this puts on f (1e-3 allowed)
+    rel_o = None
+    for F, lg in ((F0, lg0), (F1, lg1)):
+        Fo, sto, logo = oracle_on_device_path(X, R, init, lg, tie=1e-3, mttkrp_form=o.MTTKRP_DIMTREE,
+                                              cost_form=o.COST_GRAM, max_iter=25, threshold_error=0.0)
+        worst = max(abs(a["f_value"] ** 2 - b["f_value"] ** 2) for a, b in zip(lg, logo)) / frob
         rel = h.cost_explicit(F) / np.sqrt(frob)
-        say("same decisions as the oracle for", dvg, "sweeps; final rel cost", rel, "oracle", rel_o)
+        rel_o = o.cost_explicit(X, Fo) / np.sqrt(frob)
+        say("on the device's path: max |f^2 - f_oracle^2| / ||X||^2 = %.2e; final rel cost %.10f, oracle %.10f"
+            % (worst, rel, rel_o))
+        assert worst < 1e-5
         assert abs(rel * rel - rel_o * rel_o) < 1e-5
 
 
@@ -380,17 +459,12 @@ def test_time_passes_events_leave_results_unchanged(h):
 @pytest.mark.parametrize("dims,R,seed,sweeps", [([5, 4, 6, 3, 4], 3, 5, 9), ([4, 3, 5, 3, 4, 2], 3, 6, 10),
                                                 ([3, 4, 5, 3, 2, 3], 2, 7, 10)])
 def test_als_higher_order_standard_form(h, dims, R, seed, sweeps):
-    """Orders 5 and 6 through the per-mode form.  The sweep counts stop before the runs
-    converge to where f and the accelerated cost tie to ~1e-14 and the decision rests on
-    the last bits (smallest oracle margin here >= 1.4e-4 relative)."""
+    """Orders 5 and 6 through the per-mode form."""
     say("ALS order", len(dims), dims, R)
     X, _ = o.make_tensor(dims, R, seed=seed, snr_db=25.0)
     init = o.gen_factors(dims, R, seed=seed + 1)
     h.set_tensor(X)
-    kw = dict(max_iter=sweeps, threshold_error=0.0)
-    Fg, stg, logg = h.cpd(R, init, **kw)
-    Fo, sto, logo = o.cpd(X, R, init, mttkrp_form=o.MTTKRP_STANDARD, cost_form=o.COST_GRAM, **kw)
-    _compare_runs(Fg, stg, logg, Fo, sto, logo, 1e-10)
+    _als_parity(h, X, R, init, max_iter=sweeps, threshold_error=0.0)
 
 
 def _fused_equals_unfused(h, R, init, kw):
@@ -428,14 +502,7 @@ def test_als_other_constraints(h, cons):
     h.set_tensor(X)
     kw = dict(max_iter=10, threshold_error=0.0, constraints=cons)
     for form, oform in ((P.MTTKRP_STANDARD, o.MTTKRP_STANDARD), (P.MTTKRP_DIMTREE, o.MTTKRP_DIMTREE)):
-        Fg, stg, logg = h.cpd(R, init, mttkrp_form=form, **kw)
-        Fo, sto, logo = o.cpd(X, R, init, mttkrp_form=oform, cost_form=o.COST_GRAM, **kw)
-        tie = _first_accel_tie(logg, logo)
-        if tie is None:
-            _compare_runs(Fg, stg, logg, Fo, sto, logo, 1e-10)
-        else:
-            say("line-search tie at sweep", tie + 1, "- compared up to there")
-            _compare_runs(Fg, stg, logg[:tie], Fo, sto, logo[:tie], 1e-10, final=False)
+        Fg, stg, logg = _als_parity(h, X, R, init, mttkrp_form=form, oform=oform, **kw)
         for k, c in enumerate(cons):
             if c == 2:                                      # orthonormal columns
                 assert np.allclose(Fg[k].T @ Fg[k], np.eye(R), atol=1e-10)
@@ -467,14 +534,7 @@ def test_dimtree_higher_orders(h, dims, R):
     h.set_tensor(Xc, storage=P.TENSOR_BOTH)
     kw = dict(max_iter=8, threshold_error=0.0)
     _fused_equals_unfused(h, R, init, kw)
-    Fg, stg, logg = h.cpd(R, init, mttkrp_form=P.MTTKRP_DIMTREE, **kw)
-    Fo, sto, logo = o.cpd(Xc, R, init, cost_form=o.COST_GRAM, **kw)
-    tie = _first_accel_tie(logg, logo)
-    if tie is None:
-        _compare_runs(Fg, stg, logg, Fo, sto, logo, 1e-9)
-    else:
-        say("line-search tie at sweep", tie + 1, "- compared up to there")
-        _compare_runs(Fg, stg, logg[:tie], Fo, sto, logo[:tie], 1e-9, final=False)
+    _als_parity(h, Xc, R, init, mttkrp_form=P.MTTKRP_DIMTREE, **kw)
 
 
 @pytest.mark.parametrize("dims", [[6, 7, 8], [70, 37, 45]])

@@ -43,7 +43,9 @@ def test_forward_matches_oracle(h):
     Y = h.mlp_forward(X)
     Yo, _ = m.forward(w, X, bf16=True)
     say("forward rel err", P.rel_err(Y, Yo))
-    assert P.rel_err(Y, Yo) < 2e-3          # FP32 accumulation vs FP64 on the same BF16 operands
+    # FP32 accumulation vs FP64 on the same BF16 operands; a hidden activation whose FP32 and
+    # FP64 sums round to neighbouring BF16 values moves the output by ~1e-4 (measured 1.0e-4)
+    assert P.rel_err(Y, Yo) < 5e-4
 
 
 def test_gradient_matches_oracle(h):
@@ -56,7 +58,7 @@ def test_gradient_matches_oracle(h):
     for a, b in zip(BLOCKS, BLOCKS[1:]):
         err = P.rel_err(g[a:b].astype(np.float64), go[a:b])
         say("gradient block at", a, "rel err", err)
-        assert err < 5e-3
+        assert err < 1e-6                    # measured <= 9e-8: FP32 accumulation only
 
 
 def test_harvested_features_match_oracle(h):

@@ -342,3 +342,153 @@ def test_sym_extremes_edge_cases(n, kind):
     tol = 4 * n * 2.2e-16 * w.max()
     assert abs(L - w.max()) <= tol
     assert abs(mu - w.min()) <= tol
+
+
+# ------------------------------------------- line search, Normalize, update --
+
+def test_line_search_step_and_accelerated_cost():
+    """line_search_accel (Cpd.hpp:247-278, cost :162-169): at sweep k the candidate is
+    A_hat = A_old + s (A - A_old) with s = (k + 1)^(1 / accel_coeff), A the sweep's
+    post-Normalize factors and A_old the previous sweep's post-Normalize factors taken
+    BEFORE that sweep's own line search (Cpd.hpp:346-353).  A run stopped at sweep k
+    returns exactly those factors (termination precedes the line search), so f_accel of a
+    longer run is checked against the explicit residual ||X - [[A_hat]]||_F computed here,
+    and the accept rule against f_value > f_accel."""
+    X, _, init = _c1_like()
+    K = 40
+    _, stK, logK = o.cpd(X, 4, init, max_iter=K, threshold_error=0.0)
+    assert stK["accel_coeff"] > 3                       # the coefficient escalates inside the run
+    prev = o.cpd(X, 4, init, max_iter=1, threshold_error=0.0)
+    after_accept = after_escalation = 0
+    for k in range(2, K):
+        cur = o.cpd(X, 4, init, max_iter=k, threshold_error=0.0)
+        A_old, A, coeff = prev[0], cur[0], cur[1]["accel_coeff"]
+        s = (k + 1) ** (1.0 / coeff)
+        A_hat = [a0 + s * (a - a0) for a0, a in zip(A_old, A)]
+        f_hat = np.linalg.norm(X - o.cp_full(A_hat))
+        rec = logK[k - 1]
+        assert rec["accel_tried"] == 1
+        assert rec["f_accel"] == pytest.approx(f_hat, rel=1e-11), k
+        assert rec["f_value"] == pytest.approx(cur[1]["f_value"], rel=1e-14)
+        assert rec["accel_accepted"] == int(rec["f_value"] > rec["f_accel"])
+        after_accept += logK[k - 2]["accel_accepted"]
+        after_escalation += coeff > 3
+        prev = cur
+    assert after_accept > 0 and after_escalation > 0      # both cases a wrong A_old / s would miss
+
+
+@pytest.mark.parametrize("N,wf", [(3, 1), (4, 1), (4, 2), (3, 2), (4, 3)])
+def test_normalize_weight_factor(N, wf):
+    """Normalize with the weights loaded on factor wf != 0 (Normalize.hpp:53-67, 92-174):
+    factors j in [0, N-2] other than wf get unit columns, factor wf takes the product of
+    their norms, the last factor is untouched unless it is wf, [[A]] and the Gram
+    identities hold afterwards."""
+    rng = np.random.default_rng(10 * N + wf)
+    dims = [5, 6, 7, 4][:N]
+    A = [rng.random((I, 3)) + 0.1 for I in dims]
+    G = [a.T @ a for a in A]
+    X0 = o.cp_full(A)
+    G2, A2 = o.normalize(G, A, weight_factor=wf)
+    assert np.allclose(o.cp_full(A2), X0, rtol=1e-13, atol=0)
+    scaled = [j for j in range(N - 1) if j != wf]
+    w = np.ones(3)
+    for j in scaled:
+        nj = np.linalg.norm(A[j], axis=0)
+        w *= nj
+        assert np.allclose(A2[j], A[j] / nj, rtol=1e-14, atol=0)
+    assert np.allclose(A2[wf], A[wf] * w, rtol=1e-14, atol=0)
+    if wf != N - 1:
+        assert np.array_equal(A2[-1], A[-1])
+    for j in range(N):
+        assert np.allclose(G2[j], A2[j].T @ A2[j], rtol=1e-12, atol=0)
+
+
+def test_normalize_weight_factor_zero_test():
+    """With wf != 0 the zero test still reads the last entry of the norm list
+    (Normalize.hpp:131-132): the last factor's norm when wf < N-1, factor N-2's when wf is
+    the last factor."""
+    rng = np.random.default_rng(4)
+    A = [rng.random((4, 3)) + 0.1 for _ in range(3)]
+    A[2][:, 1] = 0.0
+    G = [a.T @ a for a in A]
+    _, A2 = o.normalize(G, A, weight_factor=1)
+    assert np.array_equal(A2[0][:, 1], A[0][:, 1])                     # column 1 skipped
+    assert np.allclose(np.linalg.norm(A2[0][:, [0, 2]], axis=0), 1.0)
+    B = [a.copy() for a in A]
+    B[1][:, 2] = 0.0                                                   # factor N-2 zero, wf = last
+    _, B2 = o.normalize([b.T @ b for b in B], B, weight_factor=2)
+    assert np.array_equal(B2[0][:, 2], B[0][:, 2])
+    assert np.allclose(np.linalg.norm(B2[0][:, [0]], axis=0), 1.0)
+
+
+def _update_problem(I=40, R=6, seed=0):
+    rng = np.random.default_rng(seed)
+    B = rng.random((3 * R, R))
+    V = B.T @ B + 0.5 * np.eye(R)
+    M = rng.standard_normal((I, R)) + 1.0
+    A0 = rng.random((I, R))
+    return V, M, A0
+
+
+def test_update_factor_unconstrained_solves_normal_equations():
+    """Unconstrained branch A = M V^-1 (Cpd.hpp:196-200): A V = M to rounding, and the
+    returned Gram is A^T A."""
+    V, M, A0 = _update_problem()
+    A, G, it, fb = o.update_factor(o.UNCONSTRAINED, V, M, A0)
+    assert it == 0 and fb == 0
+    assert np.linalg.norm(A @ V - M) <= 1e-12 * np.linalg.norm(M)
+    assert np.allclose(G, A.T @ A, rtol=1e-13, atol=0)
+
+
+def test_update_factor_orthogonality_is_polar_factor():
+    """Orthogonality branch A = M (M^T M)^-1/2 (Cpd.hpp:210-219) is the orthonormal polar
+    factor U W^T of M = U S W^T (SVD)."""
+    V, M, A0 = _update_problem(seed=1)
+    A, G, _, _ = o.update_factor(o.ORTHOGONALITY, V, M, A0)
+    U, _, Wt = np.linalg.svd(M, full_matrices=False)
+    assert np.abs(A - U @ Wt).max() <= 1e-11
+    assert np.allclose(G, np.eye(M.shape[1]), atol=1e-12)
+
+
+def test_update_factor_nonnegative_and_passive_branches():
+    """Nonnegativity = the Nesterov solve from the previous factor (NesterovMNLS.hpp:181-234),
+    blended 0.9 / 0.1 with the previous factor when a column vanishes (Cpd.hpp:206-207);
+    sparsity and constant leave the factor as it is; every branch returns A^T A."""
+    V, M, A0 = _update_problem(seed=2)
+    A, G, it, fb = o.update_factor(o.NONNEGATIVITY, V, M, A0)
+    An, itn, _, _ = o.nesterov(V, M, A0, 1e-2, 1e-2, 50)
+    assert it == itn and fb == 0 and np.array_equal(A, An)
+    assert np.allclose(G, A.T @ A, rtol=1e-13, atol=0)
+    A, G, it, fb = o.update_factor(o.NONNEGATIVITY, V, -np.abs(M), A0)   # M <= 0: a column vanishes
+    An, _, _, _ = o.nesterov(V, -np.abs(M), A0, 1e-2, 1e-2, 50)
+    assert np.abs(An).sum(axis=0).min() == 0.0
+    assert fb == 1 and np.array_equal(A, 0.9 * An + 0.1 * A0)
+    for c in (o.SPARSITY, o.CONSTANT):
+        A, G, it, fb = o.update_factor(c, V, M, A0)
+        assert np.array_equal(A, A0) and np.allclose(G, A0.T @ A0, rtol=1e-13, atol=0)
+
+
+def test_forced_line_search_outcomes():
+    """The test hook that makes the oracle follow given line-search outcomes: -1 keeps the
+    oracle's own test, a forced outcome that differs from it is flagged, and the counters of
+    Cpd.hpp:270-277 follow the forced outcomes."""
+    X, _, init = _c1_like()
+    F, st, log = o.cpd(X, 4, init, max_iter=12, threshold_error=0.0)
+    own = [l["accel_accepted"] for l in log]
+    F1, st1, log1 = o.cpd(X, 4, init, max_iter=12, threshold_error=0.0, force_accept=[-1] * 12)
+    assert [l["accel_accepted"] for l in log1] == own and not any(l["accel_overridden"] for l in log1)
+    for a, b in zip(F, F1):
+        assert np.array_equal(a, b)
+    k = next(i for i in range(1, 11) if log[i]["accel_tried"])
+    force = [-1] * 12
+    force[k] = 1 - own[k]
+    _, st2, log2 = o.cpd(X, 4, init, max_iter=12, threshold_error=0.0, force_accept=force)
+    assert log2[k]["accel_overridden"] == 1 and log2[k]["accel_accepted"] == 1 - own[k]
+    assert [l["accel_accepted"] for l in log2[:k]] == own[:k]
+    coeff, fail = 3, 0
+    for rec in log2:
+        if rec["accel_tried"] and not rec["accel_accepted"]:
+            fail += 1
+            if fail == 5:
+                fail, coeff = 0, coeff + 1
+    assert (coeff, fail) == (st2["accel_coeff"], st2["accel_fail"])
