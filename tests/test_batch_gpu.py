@@ -61,7 +61,10 @@ TIE = 1e-12       # |f - f_accel| / f below which a line-search outcome rests on
 
 def oracle_problem(X, R, init, logg, **kw):
     """The oracle's cpd() of one problem on the device's line-search outcomes; every sweep
-    where they differ from the oracle's own test must be a tie to TIE."""
+    where they differ from the oracle's own test must be a tie to TIE.  Also returns the
+    factor tolerance: 1e-12, or four times the spread between the oracle's own two
+    summation orders (per-mode and blocked MTTKRP on the same path) where the problem
+    amplifies rounding beyond that."""
     force = [l["accel_accepted"] if l["accel_tried"] else -1 for l in logg]
     Fo, sto, logo = o.cpd(X, R, init, force_accept=force, cost_form=o.COST_GRAM, **kw)
     for k, l in enumerate(logo):
@@ -69,7 +72,9 @@ def oracle_problem(X, R, init, logg, **kw):
             gap = abs(l["f_value"] - l["f_accel"]) / l["f_value"]
             say("line-search tie at sweep %d: oracle |f - f_accel| / f = %.2e" % (k + 1, gap))
             assert gap <= TIE, (k, l["f_value"], l["f_accel"])
-    return Fo, sto, logo
+    Fb, _, _ = o.cpd(X, R, init, force_accept=force, cost_form=o.COST_GRAM, mttkrp_form=o.MTTKRP_BLOCKED, **kw)
+    spread = max(P.rel_err(a, b) for a, b in zip(Fb, Fo))
+    return Fo, sto, logo, max(TOL, 4.0 * spread)
 
 
 def _check_problem(p, Fg, stg, logg, Fo, sto, logo, tol=TOL):
@@ -103,8 +108,10 @@ def test_batch_parity_vs_oracle(h):
     say("report", rep)
     worst = [0.0, 0.0]
     for p, ((dims, R), X) in enumerate(zip(probs, Xs)):
-        Fo, sto, logo = oracle_problem(X, R, init[p], logg[p], **kw)
-        worst = [max(a, b) for a, b in zip(worst, _check_problem(p, Fg[p], stg[p], logg[p], Fo, sto, logo))]
+        Fo, sto, logo, tol = oracle_problem(X, R, init[p], logg[p], **kw)
+        if tol > TOL:
+            say("problem", p, probs[p], "factor tolerance %.2e from the oracle's own spread" % tol)
+        worst = [max(a, b) for a, b in zip(worst, _check_problem(p, Fg[p], stg[p], logg[p], Fo, sto, logo, tol))]
         assert stg[p]["frob_sq"] == pytest.approx(sto["frob_sq"], rel=1e-13)
     say("over the batch vs oracle: max factor rel diff %.2e, max |f^2 - f_o^2| / ||X||^2 %.2e" % tuple(worst))
 
@@ -177,11 +184,11 @@ def test_batch_threshold_stop(h):
     kw = dict(max_iter=120, threshold_error=0.05)
     Fg, stg, logg, _ = h.cpd_batch(init, block=256, grid=2, **kw)
     for p, ((dims, R), X) in enumerate(zip(probs, Xs)):
-        Fo, sto, logo = oracle_problem(X, R, init[p], logg[p], **kw)
+        Fo, sto, logo, tol = oracle_problem(X, R, init[p], logg[p], **kw)
         say("problem", p, "stops at", stg[p]["ao_iter"], "oracle", sto["ao_iter"])
         assert 1 < sto["ao_iter"] < 120
         assert logg[p][-1]["stopped"] == 1 and logg[p][-1]["accel_tried"] == 0
-        _check_problem(p, Fg[p], stg[p], logg[p], Fo, sto, logo)
+        _check_problem(p, Fg[p], stg[p], logg[p], Fo, sto, logo, tol)
 
 
 def test_batch_no_accel_no_norm(h):
@@ -192,8 +199,8 @@ def test_batch_no_accel_no_norm(h):
     kw = dict(max_iter=6, threshold_error=0.0, acceleration=0, normalization=0)
     Fg, stg, logg, _ = h.cpd_batch(init, block=512, grid=4, **kw)
     for p, ((dims, R), X) in enumerate(zip(probs, Xs)):
-        Fo, sto, logo = oracle_problem(X, R, init[p], logg[p], **kw)
-        _check_problem(p, Fg[p], stg[p], logg[p], Fo, sto, logo)
+        Fo, sto, logo, tol = oracle_problem(X, R, init[p], logg[p], **kw)
+        _check_problem(p, Fg[p], stg[p], logg[p], Fo, sto, logo, tol)
 
 
 def test_batch_generator_and_init_match_oracle(h):
