@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 
 import oracle as o
+import parity_util as U
 import ptn_hip as P
 
 pytestmark = pytest.mark.gpu
@@ -60,20 +61,13 @@ TIE = 1e-12       # |f - f_accel| / f below which a line-search outcome rests on
 
 
 def oracle_problem(X, R, init, logg, **kw):
-    """The oracle's cpd() of one problem on the device's line-search outcomes; every sweep
-    where they differ from the oracle's own test must be a tie to TIE.  Also returns the
-    factor tolerance: 1e-12, or four times the spread between the oracle's own two
-    summation orders (per-mode and blocked MTTKRP on the same path) where the problem
-    amplifies rounding beyond that."""
-    force = [l["accel_accepted"] if l["accel_tried"] else -1 for l in logg]
-    Fo, sto, logo = o.cpd(X, R, init, force_accept=force, cost_form=o.COST_GRAM, **kw)
-    for k, l in enumerate(logo):
-        if l["accel_overridden"]:
-            gap = abs(l["f_value"] - l["f_accel"]) / l["f_value"]
-            say("line-search tie at sweep %d: oracle |f - f_accel| / f = %.2e" % (k + 1, gap))
-            assert gap <= TIE, (k, l["f_value"], l["f_accel"])
-    Fb, _, _ = o.cpd(X, R, init, force_accept=force, cost_form=o.COST_GRAM, mttkrp_form=o.MTTKRP_BLOCKED, **kw)
-    spread = max(P.rel_err(a, b) for a, b in zip(Fb, Fo))
+    """The oracle's cpd() of one problem on the device's line-search outcomes (every sweep
+    where they differ from the oracle's own test must be a tie to TIE), and the factor
+    tolerance: 1e-12, or four times the oracle's own spread between its summation orders
+    (per-mode, blocked and tree MTTKRP on the same path) where the problem amplifies
+    rounding beyond that (up to 1.6e-12 here)."""
+    Fo, sto, logo = U.oracle_on_device_path(X, R, init, logg, TIE, cost_form=o.COST_GRAM, **kw)
+    spread, _ = U.oracle_spread(X, R, init, logg, Fo, logo, o.MTTKRP_STANDARD, cost_form=o.COST_GRAM, **kw)
     return Fo, sto, logo, max(TOL, 4.0 * spread)
 
 

@@ -14,6 +14,7 @@ import numpy as np
 import pytest
 
 import oracle as o
+import parity_util as U
 import ptn_hip as P
 
 pytestmark = pytest.mark.gpu
@@ -239,16 +240,7 @@ TIE = 1e-12       # |f - f_accel| / f below which a line-search outcome rests on
 
 
 def oracle_on_device_path(X, R, init, logg, tie=TIE, **kw):
-    """The oracle's cpd() following the device's line-search outcomes; every sweep where
-    they differ from the oracle's own test f > f_accel must be a tie to `tie`."""
-    force = [l["accel_accepted"] if l["accel_tried"] else -1 for l in logg]
-    Fo, sto, logo = o.cpd(X, R, init, force_accept=force, **kw)
-    for k, l in enumerate(logo):
-        if l["accel_overridden"]:
-            gap = abs(l["f_value"] - l["f_accel"]) / l["f_value"]
-            say("line-search tie at sweep %d: oracle |f - f_accel| / f = %.2e" % (k + 1, gap))
-            assert gap <= tie, (k, l["f_value"], l["f_accel"])
-    return Fo, sto, logo
+    return U.oracle_on_device_path(X, R, init, logg, tie, **kw)
 
 
 def _compare_runs(Fg, stg, logg, Fo, sto, logo, tol=TOL, cost_tol=COST_TOL):
@@ -278,16 +270,13 @@ def _compare_runs(Fg, stg, logg, Fo, sto, logo, tol=TOL, cost_tol=COST_TOL):
 def _als_parity(h, X, R, init, oform=o.MTTKRP_STANDARD, **kw):
     """One device run through the C ABI against the oracle's run on its path.  The factor
     (cost) tolerance is 1e-12 (COST_TOL), or four times the spread between the oracle's own
-    two summation orders (per-mode and blocked MTTKRP, same path) where the problem
+    summation orders (per-mode, blocked and tree MTTKRP on the same path) where the problem
     amplifies rounding beyond that: the orthogonality branch's (M^T M)^-1/2 and the
     unconstrained M V^-1 reach a 2.5e-11 spread after ten sweeps in the oracle alone."""
     Fg, stg, logg = h.cpd(R, init, **kw)
     okw = {k: v for k, v in kw.items() if k != "mttkrp_form"}
     Fo, sto, logo = oracle_on_device_path(X, R, init, logg, mttkrp_form=oform, cost_form=o.COST_GRAM, **okw)
-    other = o.MTTKRP_BLOCKED if oform != o.MTTKRP_BLOCKED else o.MTTKRP_STANDARD
-    Fs, _, logs = oracle_on_device_path(X, R, init, logg, mttkrp_form=other, cost_form=o.COST_GRAM, **okw)
-    spread = max(P.rel_err(a, b) for a, b in zip(Fs, Fo))
-    cspread = max(abs(a["f_value"] ** 2 - b["f_value"] ** 2) for a, b in zip(logs, logo)) / sto["frob_sq"]
+    spread, cspread = U.oracle_spread(X, R, init, logg, Fo, logo, oform, cost_form=o.COST_GRAM, **okw)
     say("oracle's own spread between summation orders: factors %.2e, |f^2 - f^2| / ||X||^2 %.2e"
         % (spread, cspread))
     _compare_runs(Fg, stg, logg, Fo, sto, logo, max(TOL, 4.0 * spread), max(COST_TOL, 4.0 * cspread))
