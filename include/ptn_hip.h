@@ -131,6 +131,20 @@ ptn_status ptn_mttkrp_dimtree(ptn_handle h, int R, const double *factors_packed,
 /* NesterovMNLS, NesterovMNLS.hpp:181-234: Z (R x R), W, A (I x R col-major); A in/out. */
 ptn_status ptn_nesterov_mnls(ptn_handle h, int64_t I, int R, const double *Z, const double *W, double delta_1,
                              double delta_2, int max_inner, double *A, int *iters, double *L, double *mu);
+/* No counterpart in the reference: ptn_nesterov_mnls for callers that cannot allocate
+   device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+   hipMalloc).  Z (R x R), W and A (I x R, A in/out) are DEVICE pointers, column-major; the
+   results equal ptn_nesterov_mnls's bit for bit.  Every other buffer is a static device
+   array of PTN_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends, its address
+   looked up by ptn_create): a call needing more than that -- about
+   (max_inner + 4) * I * 8 * round_up(R, 16) bytes -- fails with PTN_ERR_INVALID.  All
+   calls on one GPU share that array: calls that overlap on one GPU (several handles or
+   threads) race.  The call synchronises the handle's stream before it returns;
+   iters / L / mu may be NULL. */
+#define PTN_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+ptn_status ptn_nesterov_mnls_noalloc(ptn_handle h, int64_t I, int R, const double *Z, const double *W,
+                                     double delta_1, double delta_2, int max_inner, double *A, int *iters, double *L,
+                                     double *mu);
 /* ComputeSVD extremes for the symmetric V (NesterovMNLS.hpp:45-52). */
 ptn_status ptn_sym_extremes(ptn_handle h, int R, const double *V, double *L, double *mu);
 /* Normalize, Normalize.hpp:92-174: grams (N x R x R, col-major each) and factors in/out. */

@@ -184,6 +184,13 @@ __global__ void k_cost_explicit(Dims D, int R, int ldr, FacPtrs F, int64_t total
   block_sum_store<void>(s, partials);
 }
 
+// Zero fill by vector stores (used where a buffer must start at zero in code a caller may
+// capture into a graph, instead of a memset node).
+__global__ void k_fill_zero(int64_t n, unsigned long long *p) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+    p[e] = 0ull;
+}
+
 __global__ void k_colmajor_to_rows(int64_t I, int R, int ldr, const double *src, double *dst) {
   int64_t n = I * ldr;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
@@ -256,6 +263,10 @@ void cost_explicit_partials(hipStream_t s, int N, const int64_t *dims, int R, in
   int64_t total = 1;
   for (int k = 0; k < N; ++k) total *= dims[k];
   k_cost_explicit<<<nparts, 256, 0, s>>>(D, R, ldr, F, total, x64, x32, partials);
+}
+
+void fill_zero(hipStream_t s, void *p, int64_t words) {
+  if (words > 0) k_fill_zero<<<grid_for(words, 256), 256, 0, s>>>(words, (unsigned long long *)p);
 }
 
 void colmajor_to_rows(hipStream_t s, int64_t I, int R, int ldr, const double *src, double *dst) {

@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <numeric>
 #include <random>
@@ -57,6 +58,15 @@ struct ptn_handle_s {
   BatchSet *batch = nullptr;
   MlpState *mlp = nullptr;    // the warm-start network, its training set and work buffers
 };
+
+// Device scratch of ptn_nesterov_mnls_noalloc: a file-scope device array, one copy per GPU,
+// held until the process ends.  Host code never uses the symbol itself as an address: the
+// device address of each GPU's copy is looked up once per device (ptn_create, with the
+// device current) and kept in noalloc_ptr[device].
+constexpr int kNoallocMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[PTN_NOALLOC_SCRATCH_BYTES];
+static std::mutex noalloc_mu;
+static void *noalloc_ptr[kNoallocMaxDevices] = {};
 
 namespace {
 
@@ -883,6 +893,14 @@ ptn_status ptn_create(ptn_handle *out, int device) {
     for (hipEvent_t &e : h->ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HIPCHK(hipMalloc((void **)&h->scal, 16 * sizeof(double)));
     HIPCHK(hipMalloc((void **)&h->part, kNormParts * sizeof(double)));
+    REQUIRE(device < kNoallocMaxDevices, PTN_ERR_UNSUPPORTED, "device ordinal above 63");
+    std::lock_guard<std::mutex> lock(noalloc_mu);
+    if (!noalloc_ptr[device]) {
+      void *ptr = nullptr;
+      HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+      REQUIRE(ptr != nullptr, PTN_ERR_HIP, "no device address for the no-allocation scratch");
+      noalloc_ptr[device] = ptr;
+    }
   });
   if (st != PTN_OK) {
     ptn_destroy(h);
@@ -1417,6 +1435,95 @@ ptn_status ptn_nesterov_mnls(ptn_handle h, int64_t I, int R, const double *Z, co
     if (iters) *iters = hp.iters;
     if (L) *L = hp.L0;
     if (mu) *mu = hp.mu0;
+  });
+}
+
+}  // extern "C"
+
+namespace {
+// run() without the std::function (no host heap use on the no-allocation path).
+template <typename F>
+ptn_status run_direct(ptn_handle h, F &&fn) {
+  if (!h) return PTN_ERR_INVALID;
+  try {
+    hipError_t e = hipSetDevice(h->device);
+    if (e != hipSuccess) {
+      h->err = hipGetErrorString(e);
+      return PTN_ERR_HIP;
+    }
+    fn();
+    return PTN_OK;
+  } catch (const Fail &f) {
+    return f.code;
+  } catch (...) {
+    h->err = "unexpected exception";
+    return PTN_ERR_HIP;
+  }
+}
+
+int64_t align256(int64_t bytes) { return (bytes + 255) / 256 * 256; }
+}  // namespace
+
+extern "C" {
+
+ptn_status ptn_nesterov_mnls_noalloc(ptn_handle h, int64_t I, int R, const double *Z, const double *W,
+                                     double delta_1, double delta_2, int max_inner, double *A, int *iters, double *L,
+                                     double *mu) {
+  return run_direct(h, [&] {
+    REQUIRE(R >= 1 && R <= PTN_MAX_RANK, PTN_ERR_UNSUPPORTED, "rank must be in [1, 64]");
+    REQUIRE(I >= 1 && I < (1ll << 31), PTN_ERR_INVALID, "bad row count");
+    REQUIRE(max_inner >= 0 && max_inner <= kMaxInner, PTN_ERR_UNSUPPORTED, "max_inner must be in [0, 255]");
+    REQUIRE(Z && W && A, PTN_ERR_INVALID, "null argument");
+    REQUIRE(h->device >= 0 && h->device < kNoallocMaxDevices && noalloc_ptr[h->device], PTN_ERR_STATE,
+            "no-allocation scratch not set up for this device");
+    const int ldr = round_up16(R);
+    // scratch layout, 256-byte aligned pieces: Z rows, W rows, A rows, result rows, the
+    // iterates, betas, column sums, NnlsParams, satisfaction masks
+    const int64_t rows = I * ldr * (int64_t)sizeof(double);
+    const int64_t o_z = 0, o_w = o_z + align256((int64_t)R * R * sizeof(double)), o_a = o_w + align256(rows),
+                  o_b = o_a + align256(rows), o_hist = o_b + align256(rows),
+                  o_beta = o_hist + align256((int64_t)(max_inner + 1) * rows),
+                  o_cs = o_beta + align256((kMaxInner + 1) * (int64_t)sizeof(double)),
+                  o_prm = o_cs + align256(kMaxRank * (int64_t)sizeof(double)),
+                  o_mask = o_prm + align256((int64_t)sizeof(NnlsParams)),
+                  end = o_mask + align256(nnls_mask_words(I) * (int64_t)sizeof(uint64_t));
+    if (end > (int64_t)PTN_NOALLOC_SCRATCH_BYTES) {
+      h->err = "ptn_nesterov_mnls_noalloc: I = " + std::to_string(I) + ", R = " + std::to_string(R) +
+               ", max_inner = " + std::to_string(max_inner) + " needs " + std::to_string(end) +
+               " bytes of scratch, above the PTN_NOALLOC_SCRATCH_BYTES limit of " +
+               std::to_string((long long)PTN_NOALLOC_SCRATCH_BYTES) + "; use ptn_nesterov_mnls";
+      throw Fail{PTN_ERR_INVALID};
+    }
+    unsigned char *base = (unsigned char *)noalloc_ptr[h->device];
+    double *dZ = (double *)(base + o_z), *dW = (double *)(base + o_w), *dA = (double *)(base + o_a),
+           *dB = (double *)(base + o_b), *hist = (double *)(base + o_hist), *betas = (double *)(base + o_beta),
+           *colsum = (double *)(base + o_cs);
+    NnlsParams *prm = (NnlsParams *)(base + o_prm);
+    uint64_t *mask = (uint64_t *)(base + o_mask);
+    hipStream_t s = h->stream;
+    // the scratch keeps the previous call's contents: betas, column sums, NnlsParams and
+    // masks start at zero as in ptn_nesterov_mnls (every other piece is written before it
+    // is read)
+    fill_zero(s, base + o_beta, (end - o_beta) / 8);
+    colmajor_to_rows(s, R, R, R, Z, dZ);            // row-major Z (Z[a + R b] -> dZ[a R + b])
+    colmajor_to_rows(s, I, R, ldr, W, dW);
+    colmajor_to_rows(s, I, R, ldr, A, dA);
+    eig_extremes(s, R, dZ, prm);
+    nnls_prep(s, prm, max_inner, betas);
+    nnls_iterate(s, I, R, ldr, dZ, prm, betas, dW, dA, nullptr, delta_1, delta_2, max_inner, mask, hist, colsum);
+    nnls_select(s, I, R, ldr, hist, mask, max_inner, dB, colsum, prm);
+    rows_to_colmajor(s, I, R, ldr, dB, A);
+    HIPCHK(hipGetLastError());
+    if (iters || L || mu) {
+      NnlsParams hp;
+      HIPCHK(hipMemcpyAsync(&hp, prm, sizeof(hp), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      if (iters) *iters = hp.iters;
+      if (L) *L = hp.L0;
+      if (mu) *mu = hp.mu0;
+    } else {
+      HIPCHK(hipStreamSynchronize(s));
+    }
   });
 }
 

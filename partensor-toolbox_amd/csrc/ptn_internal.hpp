@@ -143,6 +143,8 @@ void cost_explicit_partials(hipStream_t s, int N, const int64_t *dims, int R, in
                             const double *x64, const float *x32, double *partials, int nparts);
 
 // ---- layout conversion ----
+// zeroes `words` 8-byte words at p with a kernel (no memset node)
+void fill_zero(hipStream_t s, void *p, int64_t words);
 void colmajor_to_rows(hipStream_t s, int64_t I, int R, int ldr, const double *src, double *dst);
 void rows_to_colmajor(hipStream_t s, int64_t I, int R, int ldr, const double *src, double *dst);
 void rows_to_f32(hipStream_t s, int64_t n, const double *src, float *dst);

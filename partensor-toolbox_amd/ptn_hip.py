@@ -27,7 +27,8 @@ ARITH_F64, ARITH_F32, ARITH_BF16X3, ARITH_F32_F64 = 0, 1, 2, 3
 EXPORTED = [
     "ptn_create", "ptn_destroy", "ptn_last_error", "ptn_default_options", "ptn_abi_version",
     "ptn_set_tensor", "ptn_generate_tensor", "ptn_generate_factors", "ptn_tensor_norm_sq",
-    "ptn_cpd", "ptn_mttkrp", "ptn_mttkrp_dimtree", "ptn_nesterov_mnls", "ptn_sym_extremes",
+    "ptn_cpd", "ptn_mttkrp", "ptn_mttkrp_dimtree", "ptn_nesterov_mnls", "ptn_nesterov_mnls_noalloc",
+    "ptn_sym_extremes",
     "ptn_normalize", "ptn_cost_explicit", "ptn_time_contraction",
     "ptn_batch_set", "ptn_batch_generate", "ptn_batch_init_factors", "ptn_batch_get_tensor", "ptn_cpd_batch",
     "ptn_mlp_init", "ptn_mlp_set_params", "ptn_mlp_get_params", "ptn_mlp_forward", "ptn_mlp_loss_grad",
@@ -107,6 +108,8 @@ def lib():
         L.ptn_mttkrp.argtypes = [_vp, _i, _pd, _i, _i, _pd]
         L.ptn_mttkrp_dimtree.argtypes = [_vp, _i, _pd, _i, _pd]
         L.ptn_nesterov_mnls.argtypes = [_vp, _i64, _i, _pd, _pd, _d, _d, _i, _pd, ctypes.POINTER(_i), _pd, _pd]
+        L.ptn_nesterov_mnls_noalloc.argtypes = [_vp, _i64, _i, _vp, _vp, _d, _d, _i, _vp, ctypes.POINTER(_i), _pd,
+                                                _pd]
         L.ptn_sym_extremes.argtypes = [_vp, _i, _pd, _pd, _pd]
         L.ptn_normalize.argtypes = [_vp, _i, _pi64, _i, _i, _pd, _pd]
         L.ptn_cost_explicit.argtypes = [_vp, _i, _pd, _pd]
@@ -409,6 +412,14 @@ class Handle:
         self._check(lib().ptn_nesterov_mnls(self._h, A.shape[0], A.shape[1], _p(Z), _p(W), d1, d2, max_inner, _p(A),
                                             ctypes.byref(it), ctypes.byref(L), ctypes.byref(mu)))
         return A, it.value, L.value, mu.value
+
+    def nesterov_noalloc(self, I, R, Z_dev, W_dev, A_dev, d1=1e-2, d2=1e-2, max_inner=50):
+        """ptn_nesterov_mnls_noalloc on device buffers (addresses as ints): Z (R x R), W and A
+        (I x R, A in/out), column-major.  Returns (iters, L, mu)."""
+        it, L, mu = _i(), _d(), _d()
+        self._check(lib().ptn_nesterov_mnls_noalloc(self._h, I, R, Z_dev, W_dev, d1, d2, max_inner, A_dev,
+                                                    ctypes.byref(it), ctypes.byref(L), ctypes.byref(mu)))
+        return it.value, L.value, mu.value
 
     def sym_extremes(self, V):
         V = np.asfortranarray(V, dtype=np.float64)
