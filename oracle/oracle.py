@@ -14,7 +14,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libptn_oracle.so")
+# PTN_ORACLE_LIB selects another build of the same source (the sanitised one under test)
+LIB_PATH = os.environ.get("PTN_ORACLE_LIB") or os.path.join(_HERE, "libptn_oracle.so")
 MAX_ORDER = 8
 
 UNCONSTRAINED, NONNEGATIVITY, ORTHOGONALITY, SPARSITY, CONSTANT = 0, 1, 2, 3, 4
