@@ -10,11 +10,14 @@ run does exactly W + K sweeps (30 by default, the configuration's count).
 A step is one ALS sweep.  The timed region is sweeps W+1 .. W+K of one
 decomposition (per-sweep device-synchronised timestamps from the library);
 inputs are resident in HBM before it starts.  `value` is the chosen form's
-sweeps/s.  Also reported: the plain FP64 GPU path (per-mode MTTKRP, FP64
-tensor and arithmetic) on the same tensor and initial factors, both final
-relative errors measured against the FP64 tensor, the dominant contraction's
-HBM roofline (HIP events on the library stream), and the FP64 CPU oracle
-(OpenMP restatement of cpd_omp) timed on a bounded sample on the host cores.
+sweeps/s, the median of three runs of the timed region.  Also reported: the
+plain FP64 GPU paths (per-mode MTTKRP as cpd, and the dimension tree as
+cpdDimTree; FP64 tensor and arithmetic) on the same tensor and initial factors,
+the final relative errors measured against the FP64 tensor, the chosen factors'
+relative difference from the plain ones, the dominant contraction's HBM
+roofline (HIP events on the library stream), and the FP64 CPU oracle (OpenMP
+restatement of cpd_omp) timed on a bounded sample on the host cores, whose
+four-sweep factors the device's four-sweep runs are also compared with.
 """
 import argparse
 import json
@@ -34,6 +37,7 @@ METRIC = "ALS sweeps/s + MTTKRP GB/s (800³,R=32): chosen vs plain GPU @ matched
 DIMS = [800, 800, 800]
 R = 32
 HBM_PEAK_GBS = 8000.0
+REPEATS = 3          # the timed region is run this many times; value is the median
 
 
 def log(*a):
@@ -73,24 +77,82 @@ def pass_times(sweep_log, W, K):
     return out
 
 
-def cpu_baseline(sweeps=8):
-    """FP64 OpenMP restatement of cpd_omp (CpdOpenMP.hpp:266-329, blocked MTTKRP
-    MTTKRP.hpp:102-225, explicit cost) on the same 800^3 workload, a bounded number
-    of sweeps (about 10 s on 16 cores)."""
+def cpu_info():
+    """CPU model, logical CPUs this process may run on, and the physical cores among them."""
+    model, phys = "unknown", {}
+    try:
+        cur = {}
+        for line in open("/proc/cpuinfo"):
+            k, sep, v = line.partition(":")
+            if not sep:
+                if "processor" in cur:
+                    phys[int(cur["processor"])] = (cur.get("physical id", "0"), cur.get("core id", cur["processor"]))
+                cur = {}
+                continue
+            cur[k.strip()] = v.strip()
+            if k.strip() == "model name":
+                model = v.strip()
+        if "processor" in cur:
+            phys[int(cur["processor"])] = (cur.get("physical id", "0"), cur.get("core id", cur["processor"]))
+    except OSError:
+        pass
+    avail = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else list(range(os.cpu_count() or 1))
+    cores = len({phys[c] for c in avail if c in phys}) or len(avail)
+    return model, len(avail), cores
+
+
+def cpu_baseline(device_runs):
+    """The FP64 OpenMP restatement of cpd_omp (CpdOpenMP.hpp:266-329: blocked MTTKRP
+    MTTKRP.hpp:102-225, explicit cost CpdOpenMP.hpp:121-125) on the host cores, bounded:
+    the 800^3 workload for 1 warm-up + 3 timed sweeps, three times (median), the same with
+    the Gram-form cost once, and the 100^3 R=10 configuration for 1 + 49 sweeps.  The
+    four-sweep runs also give the oracle's factors that the device's four-sweep runs
+    (device_runs: name -> factors) are compared with."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")      # read when the OpenMP runtime loads
+    os.environ.setdefault("OMP_PLACES", "cores")
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as o
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    model, logical, physical = cpu_info()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or physical
     t = time.time()
     X, _ = o.make_tensor(DIMS, R, seed=0, snr_db=20.0)
     init = o.gen_factors(DIMS, R, seed=1)
-    log("cpu baseline: tensor generated in %.1f s, %d threads" % (time.time() - t, threads))
-    F, st, lg = o.cpd(X, R, init, max_iter=sweeps, threshold_error=0.0, mttkrp_form=o.MTTKRP_BLOCKED,
-                      cost_form=o.COST_EXPLICIT, threads=threads)
-    secs = lg[-1]["t_end"] - lg[0]["t_start"]
+    frob = float(np.sum(X * X))
+    log("cpu baseline: %s, %d logical CPUs / %d physical cores available, %d threads, tensor in %.1f s"
+        % (model, logical, physical, threads, time.time() - t))
+
+    def sweeps_per_s(cost_form):
+        F, st, lg = o.cpd(X, R, init, max_iter=4, threshold_error=0.0, mttkrp_form=o.MTTKRP_BLOCKED,
+                          cost_form=cost_form, threads=threads)
+        return 3.0 / (lg[3]["t_end"] - lg[0]["t_end"]), F, st
+    reps = []
+    Fo = sto = None
+    for _ in range(3):
+        v, Fo, sto = sweeps_per_s(o.COST_EXPLICIT)
+        reps.append(v)
+    gram_v, _, _ = sweeps_per_s(o.COST_GRAM)
+    vs_oracle = {}
+    for name, F in device_runs.items():
+        vs_oracle[name] = {
+            "factor_rel_err": max(P.rel_err(a, b) for a, b in zip(F, Fo)),
+            "rel_error_gap": abs(o.cost_explicit(X, F) - o.cost_explicit(X, Fo)) / math.sqrt(frob)}
     del X
-    return {"value": sweeps / secs, "unit": "sweeps/s", "cores": threads, "kind": "port",
-            "sample": "%d full sweeps (acceleration from the second) of the 800^3 R=32 FP64 workload, "
-                      "oracle cpd_omp restatement: blocked MTTKRP + explicit cost, %.1f s" % (sweeps, secs)}
+    X1, _ = o.make_tensor([100, 100, 100], 10, seed=0, snr_db=30.0)
+    init1 = o.gen_factors([100, 100, 100], 10, seed=1)
+    _, _, lg1 = o.cpd(X1, 10, init1, max_iter=50, threshold_error=0.0, mttkrp_form=o.MTTKRP_BLOCKED,
+                      cost_form=o.COST_EXPLICIT, threads=threads)
+    c1 = 49.0 / (lg1[49]["t_end"] - lg1[0]["t_end"])
+    med = sorted(reps)[1]
+    return {"value": med, "unit": "sweeps/s", "cores": threads, "kind": "port",
+            "sample": "800^3 R=32 FP64 workload, oracle cpd_omp restatement (blocked MTTKRP + explicit cost): "
+                      "1 warm-up + 3 timed sweeps, median of 3 runs (%s sweeps/s); Gram-form cost variant %.4f "
+                      "sweeps/s; 100^3 R=10 (50 sweeps, 49 timed) %.2f sweeps/s"
+                      % (", ".join("%.4f" % v for v in reps), gram_v, c1),
+            "cpu_model": model, "logical_cpus": logical, "physical_cores": physical,
+            "omp": "OMP_NUM_THREADS=%d OMP_PROC_BIND=%s OMP_PLACES=%s"
+                   % (threads, os.environ["OMP_PROC_BIND"], os.environ["OMP_PLACES"]),
+            "runs": reps, "gram_cost_variant": gram_v, "c1_100cubed_R10": c1,
+            "device_vs_oracle_after_4_sweeps": vs_oracle}
 
 
 def copy_calibration(nbytes):
@@ -401,18 +463,37 @@ def main():
         arith = P.ARITH_F32_F64
         chosen_kw.update(arith=arith, fuse_accel=0)
         Fc, stc, lgc = h.cpd(R, init, **chosen_kw)
-    tc = timed_sweeps(stc, lgc, W, K)
+    # the timed region (sweeps W+1 .. W+K) three times; `value` is the median
+    runs = [K / timed_sweeps(stc, lgc, W, K)]
+    for _ in range(REPEATS - 1):
+        _, st_r, lg_r = h.cpd(R, init, **chosen_kw)
+        runs.append(K / timed_sweeps(st_r, lg_r, W, K))
+    value = sorted(runs)[len(runs) // 2]
     err_c = h.cost_explicit(Fc) / math.sqrt(frob64)
-    log("chosen: %.1f sweeps/s, rel err vs FP64 X %.9e" % (K / tc, err_c))
+    log("chosen: %s sweeps/s (median %.1f), rel err vs FP64 X %.9e"
+        % (", ".join("%.1f" % v for v in runs), value, err_c))
 
-    plain = None
+    plain = plain_tree = None
     if not args.no_plain:
-        Fp, stp, lgp = h.cpd(R, init, mttkrp_form=P.MTTKRP_STANDARD, arith=P.ARITH_F64, **common)
-        tp = timed_sweeps(stp, lgp, W, K)
-        err_p = h.cost_explicit(Fp) / math.sqrt(frob64)
-        plain = {"sweeps_per_s": K / tp, "ms_per_sweep": 1e3 * tp / K, "final_rel_error": err_p,
-                 "form": "per-mode MTTKRP, FP64 tensor, FP64 matrix cores, Gram-form FP64 cost"}
-        log("plain: %.1f sweeps/s, rel err %.9e" % (K / tp, err_p))
+        def plain_run(form, label):
+            Fp, stp, lgp = h.cpd(R, init, mttkrp_form=form, arith=P.ARITH_F64, **common)
+            tp = timed_sweeps(stp, lgp, W, K)
+            err_p = h.cost_explicit(Fp) / math.sqrt(frob64)
+            same = [l["accel_accepted"] for l in lgp] == [l["accel_accepted"] for l in lgc]
+            log("%s: %.1f sweeps/s, rel err %.9e" % (label, K / tp, err_p))
+            return Fp, {"sweeps_per_s": K / tp, "ms_per_sweep": 1e3 * tp / K, "final_rel_error": err_p,
+                        "chosen_factor_rel_err": max(P.rel_err(a, b) for a, b in zip(Fc, Fp)),
+                        "same_line_search_path_as_chosen": same}
+        Fp, plain = plain_run(P.MTTKRP_STANDARD, "plain per-mode")
+        plain["form"] = "per-mode MTTKRP (cpd), FP64 tensor, FP64 matrix cores, Gram-form FP64 cost"
+        _, plain_tree = plain_run(P.MTTKRP_DIMTREE, "plain tree")
+        plain_tree["form"] = "dimension tree (cpdDimTree), FP64 tensor, FP64 matrix cores, Gram-form FP64 cost"
+
+    # four-sweep runs for the comparison with the CPU oracle's four sweeps (CPU leg)
+    short = dict(common, max_iter=4)
+    dev4 = {"chosen": h.cpd(R, init, **dict(chosen_kw, max_iter=4))[0]}
+    if not args.no_plain:
+        dev4["plain_per_mode"] = h.cpd(R, init, mttkrp_form=P.MTTKRP_STANDARD, arith=P.ARITH_F64, **short)[0]
 
     # roofline of the dominant kernel, measured live: the same run again with HIP events
     # around the two tensor passes of every sweep (on the streams they are launched on).
@@ -438,11 +519,12 @@ def main():
         % (ms_l, gbs_l, ms_r, gbs_r))
     h.close()
 
-    cpu = None if args.no_cpu else cpu_baseline()
+    cpu = None if args.no_cpu else cpu_baseline(dev4)
     if cpu:
-        log("cpu baseline %.4f sweeps/s on %d threads" % (cpu["value"], cpu["cores"]))
+        log("cpu baseline %.4f sweeps/s on %d threads; device vs oracle after 4 sweeps: %s"
+            % (cpu["value"], cpu["cores"], cpu["device_vs_oracle_after_4_sweeps"]))
 
-    value = K / tc
+    tc = K / value
     out = {
         "metric": METRIC,
         "value": value,
@@ -463,8 +545,11 @@ def main():
                            % {P.ARITH_BF16X3: "bf16x3 MFMA", P.ARITH_F32: "FP32 MFMA",
                               P.ARITH_F32_F64: "FP64 MFMA on FP32 data"}[arith],
             "chosen_final_rel_error": err_c,
+            "chosen_runs_sweeps_per_s": runs,
             "plain_gpu": plain,
+            "plain_gpu_tree": plain_tree,
             "speedup_vs_plain": (value / plain["sweeps_per_s"]) if plain else None,
+            "speedup_vs_plain_tree": (value / plain_tree["sweeps_per_s"]) if plain_tree else None,
             "rel_error_gap": abs(err_c - plain["final_rel_error"]) if plain else None,
             "mttkrp_gbs": {"left_node_pass_in_sweep": gbs_p, "right_node_pass_in_sweep": gbs_rn,
                            "left_node_alone": gbs_l, "right_node_fused_alone": gbs_r},
@@ -473,6 +558,9 @@ def main():
                      "frac": gbs_p / HBM_PEAK_GBS,
                      "traffic": measured_traffic("left_pair_traffic_bytes" if pair_cols == 2 * R
                                                  else "left_node_traffic_bytes"),
+                     "traffic_source": "calibration committed in profiles/pmc_traffic_tensor_passes_800cubed_R32.md "
+                                       "(rocprofv3 FETCH_SIZE + WRITE_SIZE of this kernel scaled by a plain copy's "
+                                       "readings), not measured in this run",
                      "kernel": "k_gemm_cm_f32, the sweep's left-node pass X x_2 [A_2 | Ahat_2] (paired with the "
                                "line search)" if pair_cols == 2 * R else "k_gemm_cm_f32, left-node pass X x_2 A_2",
                      "avg_ms": ms_p, "launches_timed": n_p, "algorithmic_bytes": bytes_p,

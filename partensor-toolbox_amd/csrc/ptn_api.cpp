@@ -7,6 +7,7 @@
 #include "ptn_internal.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <functional>
 #include <cmath>
@@ -1125,8 +1126,8 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
     ModeIn sel[kMaxOrder];
     bool prepped0 = false;
     // The host enqueues sweep it + 1 before it reads sweep it back (page-locked buffers
-    // the kernels write directly, one event per sweep), so the device never waits for
-    // the host.  The line-search
+    // the kernels write directly; the last of them publishes the sweep index), so the
+    // device never waits for the host.  The line-search
     // decision and its counters live on the device (flags[0..3]); a sweep that ends the
     // run on the threshold is found one sweep late, and the factors come from its
     // snapshot.
@@ -1134,6 +1135,7 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
       NnlsParams hp[kMaxOrder + 1];
       double f, fa;
       int flags[4];
+      unsigned seq;                           // the sweep index, published last by the device
     };
     PinnedBuf<Readback> rbuf0(h), rbuf1(h);
     struct PassEvents {                       // time_passes: 2 sweep slots x (start, end) x 2 passes
@@ -1193,6 +1195,10 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
       ex.words[0] = (int)(sizeof(NnlsParams) * (N + 1) / 4);
       ex.src[1] = (const unsigned *)(w.scal.p + 1); ex.dst[1] = (unsigned *)&rbdev->f; ex.words[1] = 2;
       ex.count = side_cost ? 1 : 2;    // with side_cost the decision exports the cost
+      // the last kernel of the sweep that writes the readback publishes the sweep index
+      // (untimed runs: the host waits for it instead of an event at the end of the sweep)
+      const bool will_try = !cap && o.acceleration && it > 1;
+      if (!will_try && !w.kev) { ex.seq_dst = &rbdev->seq; ex.seq = (unsigned)it; }
       // line_search_accel, Cpd.hpp:247-278: step (k+1)^(1/accel_coeff); the device's
       // coefficient is c_known or, after one more fifth failure, c_known + 1.  A_hat =
       // A_old + step (A - A_old); A_old = A (the post-Normalize factors) and the sweep's
@@ -1263,6 +1269,7 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
           cp.fa_out = &rbdev->fa;
           cp.flags_out = rbdev->flags;
           cp.f_out = side_cost ? &rbdev->f : nullptr;
+          if (!w.kev) { cp.seq_dst = &rbdev->seq; cp.seq = (unsigned)it; }
           accel_decide(ms, w.scal.p, o.threshold_error, w.flags, cp);
           if (pair) {
             // the next sweep's left node: the half of the pair the device kept
@@ -1281,12 +1288,29 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
         }
       }
       if (!snapped) HIPCHK(hipMemcpyAsync(snaps[it & 1], w.fac, w.fsz * sizeof(double), hipMemcpyDeviceToDevice, ms));
-      HIPCHK(hipEventRecord(evs.e[it & 1], ms));
+      if (w.kev) HIPCHK(hipEventRecord(evs.e[it & 1], ms));   // timed runs read their pass events after it
       tried_at[it & 1] = tried;
     };
 
     auto process = [&](int it) {
-      HIPCHK(hipEventSynchronize(evs.e[it & 1]));
+      if (w.kev) {
+        HIPCHK(hipEventSynchronize(evs.e[it & 1]));
+      } else {
+        // the sweep's last readback kernel publishes `it` after a system-scope fence; the
+        // stream is queried now and then so that a failed launch ends the wait
+        const volatile unsigned *sq = &rbs[it & 1]->seq;
+        for (unsigned spin = 1; *sq != (unsigned)it; ++spin) {
+          if ((spin & 255) == 0) {
+            const hipError_t q = hipStreamQuery(ms);
+            if (q == hipSuccess && *sq != (unsigned)it) {
+              h->err = "sweep readback missing after the stream completed";
+              throw Fail{PTN_ERR_HIP};
+            }
+            if (q != hipSuccess && q != hipErrorNotReady) HIPCHK(q);
+          }
+        }
+        std::atomic_thread_fence(std::memory_order_acquire);
+      }
       const Readback *rb = rbs[it & 1];
       ptn_sweep_log lg{};
       r.ao_iter = it;

@@ -224,12 +224,16 @@ void nnls_finish(hipStream_t s, int64_t I, int R, int ldr, const double *hist, c
 void cost_gram(hipStream_t s, int64_t nma, const double *M, const double *A, int R, const double *V, const double *G,
                const double *frob, double *f, int accel);
 
-// up to four ranges of 4-byte words copied to mapped host memory by one kernel
+// up to four ranges of 4-byte words copied to mapped host memory by one kernel; with
+// seq_dst the kernel then publishes seq there (after a system-scope fence), telling the
+// host that the readback of the sweep is complete
 struct ExportList {
   const unsigned *src[4];
   unsigned *dst[4];
   int words[4];
   int count;
+  unsigned *seq_dst;
+  unsigned seq;
 };
 void export_words(hipStream_t s, const ExportList &x);
 // the line-search step fused into Normalize's row pass (k_axpy_accel's arithmetic); cur is
@@ -261,6 +265,8 @@ struct AccelCopy {
   double *fa_out;
   int *flags_out;
   double *f_out;                             // optional: the sweep's cost
+  unsigned *seq_dst;                         // optional: published last, as in ExportList
+  unsigned seq;
 };
 void accel_decide(hipStream_t s, const double *scal, double thr, int *flags, const AccelCopy &cp);
 void axpy_accel(hipStream_t s, int64_t n, double step_c, double step_c1, int c, const int *coeff, double *old,

@@ -406,10 +406,21 @@ struct NormArgs {
 
 // Column norms, scales and the Gram rescaling of Normalize (one block).  Columns whose
 // norms vanish keep scale 1 on every factor, so the row pass leaves them bit-identical.
+// After every thread of the block has stored its part of a readback to mapped host
+// memory: one thread fences at system scope and stores the sequence word (vector store).
+__device__ __forceinline__ void publish_seq(unsigned *dst, unsigned seq) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence_system();
+    *(volatile unsigned *)dst = seq;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_normalize(NormArgs na, double *scratch, ExportList ex) {
   // the sweep's readback first (it does not depend on Normalize)
   for (int q = 0; q < ex.count; ++q)
     for (int e = threadIdx.x; e < ex.words[q]; e += blockDim.x) ex.dst[q][e] = ex.src[q][e];
+  if (ex.seq_dst) publish_seq(ex.seq_dst, ex.seq);
   const int N = na.N, R = na.R, last = N - 1;
   double *lam = scratch;                       // (N-1) x R
   double *scale = scratch + kMaxOrder * kMaxRank;  // N x R column scales (1 = untouched)
@@ -505,6 +516,7 @@ __global__ void k_axpy_accel(int64_t n, double step_c, double step_c1, int c, co
 __global__ void k_export(ExportList x) {
   for (int q = 0; q < x.count; ++q)
     for (int e = threadIdx.x; e < x.words[q]; e += blockDim.x) x.dst[q][e] = x.src[q][e];
+  if (x.seq_dst) publish_seq(x.seq_dst, x.seq);
 }
 
 // Line-search decision on the device (Cpd.hpp:262-269): accept the accelerated factors
@@ -533,6 +545,10 @@ __global__ void k_accel_decide(const double *__restrict__ scal, double thr, int 
       for (int q = 0; q < 4; ++q) cp.flags_out[q] = flags[q];
     }
     if (cp.f_out) *cp.f_out = f;
+    if (cp.seq_dst) {                        // the sweep's readback is complete
+      __threadfence_system();
+      *(volatile unsigned *)cp.seq_dst = cp.seq;
+    }
   }
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   if (cp.pick_n) {
