@@ -187,6 +187,8 @@ typedef struct {
   int    sweeps;       /* sweeps run (every problem stops on its own threshold test before) */
   double seconds;      /* device time of all sweeps (HIP events around the graph launches) */
   double tune_seconds; /* wall time of the autotuner (not part of seconds) */
+  int    split_problems; /* problems whose MTTKRP sums are cut in pieces that several blocks share */
+  int    max_pieces;     /* the largest piece count (at most 8) */
 } ptn_batch_report;
 
 /* The batch: dims [nprob][3], ranks [nprob]; X = the tensors back to back, each
@@ -206,7 +208,7 @@ ptn_status ptn_batch_get_tensor(ptn_handle h, int p, double *X_out);
    cores, PTN_ARITH_F32 on the FP32 matrix cores with FP64 totals every 12 k-steps, all
    else FP64 either way).  init_packed / out_packed: factors packed per problem,
    problems back to back; res: nprob records; log: NULL or [nprob][opt->max_iter].
-   block / grid: launch shape of the sweep kernel (256 or 512 threads), or both 0 to let
+   block / grid: launch shape of the sweep kernel (256 or 512 threads, grid <= 8192), or both 0 to let
    the autotuner time a few shapes on the first two sweeps (state restored afterwards).
    rep: NULL or the shape used and the times. */
 ptn_status ptn_cpd_batch(ptn_handle h, const double *init_packed, const ptn_options *opt, double *out_packed,

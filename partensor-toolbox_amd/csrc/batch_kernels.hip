@@ -138,21 +138,22 @@ struct MfmaPart {
 // instruction touches are consecutive floats.
 template <bool F32, int LAST_Q>
 __device__ __forceinline__ void mttkrp_mfma(const float *X, int Iout, int Ia, int Ib, const double *Fa,
-                                            const double *Fb, double *M, int nw, MfmaPart &part) {
+                                            const double *Fb, double *M, int nw, int64_t sb, int64_t se,
+                                            MfmaPart &part) {
   const int t = tid(), lane = t & 63, wave = t >> 6;
   part.band0 = part.band1 = -1;
   if (wave >= nw) return;
-  const int K = Ia * Ib, KS = (K + 3) / 4, KF = K / 4, NB = (Iout + 31) / 32;
+  const int K = Ia * Ib, KS = (K + 3) / 4, KF = K / 4;
   constexpr int rs = 1;
-  const int64_t S = (int64_t)NB * KS;
+  const int64_t S = se - sb;                 // the steps [sb, se) of the (band, k-step) space
   int64_t s0, s1;
   if constexpr (LAST_Q == 4) {
-    s0 = (int64_t)wave * S / nw;
-    s1 = (int64_t)(wave + 1) * S / nw;
+    s0 = sb + (int64_t)wave * S / nw;
+    s1 = sb + (int64_t)(wave + 1) * S / nw;
   } else {
     const int64_t den = 4 * (nw - 1) + LAST_Q;
-    s0 = 4 * wave * S / den;
-    s1 = wave + 1 >= nw ? S : 4 * (wave + 1) * S / den;
+    s0 = sb + 4 * wave * S / den;
+    s1 = wave + 1 >= nw ? se : sb + 4 * (wave + 1) * S / den;
   }
   const int m = lane & 15, kk = lane >> 4;
   const int inc_a = 4 % Ia, inc_b = 4 / Ia;      // k += 4 as (ia, ib) += (inc_a, inc_b), one wrap at most
@@ -312,27 +313,105 @@ __device__ __forceinline__ void mttkrp_mfma_reduce(int Iout, int nw, const doubl
   }
 }
 
+// The blocks running one problem (BatchDesc::V, G): this block's member index, the
+// problem's exchange area and arrival counter, and the exchanges done so far (the same
+// count in every member).
+struct Group {
+  int g, G, V;
+  double *mx;
+  int *ctr, *err;
+  int xi;
+};
+
+// Arrival at a group exchange: every thread makes its stores visible at device scope, one
+// thread adds the block's arrival and waits until `target` arrivals are counted (bounded:
+// after ~2^20 polls, or once any wait has run out, it gives up and sets *err), and every
+// thread's acquire fence then keeps stale L1 lines out of the reads that follow.
+__device__ __forceinline__ void group_wait(int *ctr, int target, int *err) {
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicAdd(ctr, 1);
+    for (int spin = 0; __hip_atomic_load(ctr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target; ++spin) {
+      if (spin >= (1 << 20) || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        atomicExch(err, 1);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+  }
+  __syncthreads();
+  __threadfence();
+}
+
+// does the step range [sb, se) touch band b (steps [b KS, (b + 1) KS))
+__device__ __forceinline__ bool piece_touches(int64_t sb, int64_t se, int KS, int b) {
+  return sb < se && sb < (int64_t)(b + 1) * KS && se > (int64_t)b * KS;
+}
+
 // one mode's MTTKRP into S.M with operands S.F[0], S.F[1] on waves [0, nw).  LAST_Q = 4: the
 // waves past them run `other` meanwhile; LAST_Q < 4: the last of the nw waves runs `other`
-// first and then LAST_Q quarters of a share.  P holds 2 x 32 x 16 doubles per wave.
+// first and then LAST_Q quarters of a share.  P holds 2 x 32 x 16 doubles per wave.  With
+// grp.V > 1 the (band, k-step) space is cut in V contiguous pieces, each summed by the
+// block's waves; this member computes pieces [g V / G, (g + 1) V / G), stores each piece's
+// bands in the exchange area, waits for the other members and adds the pieces in order, so
+// every member holds the same M, independent of G.
 template <int BS, bool F32, int LAST_Q, typename Other>
 __device__ __forceinline__ void mttkrp_block(const float *X, int Iout, int Ia, int Ib, BatchLds &S, double *P, int nw,
-                                             Other &&other) {
-  MfmaPart part;
-  if constexpr (LAST_Q == 4) {
-    if ((int)(tid() >> 6) < nw) mttkrp_mfma<F32, 4>(X, Iout, Ia, Ib, S.F[0], S.F[1], S.M, nw, part);
-    else {
-      part.band0 = part.band1 = -1;
-      other();
+                                             Group &grp, Other &&other) {
+  const int K = Ia * Ib, KS = (K + 3) / 4, NB = (Iout + 31) / 32;
+  const int64_t Stot = (int64_t)NB * KS;
+  const int V = grp.V, v0 = grp.g * V / grp.G, v1 = (grp.g + 1) * V / grp.G;
+  double *buf = V > 1 ? grp.mx + (int64_t)(grp.xi & 1) * V * kBMax * kBR : nullptr;
+  for (int v = v0; v < v1; ++v) {
+    const int64_t sb = (int64_t)v * Stot / V, se = (int64_t)(v + 1) * Stot / V;
+    MfmaPart part;
+    if constexpr (LAST_Q == 4) {
+      if ((int)(tid() >> 6) < nw) mttkrp_mfma<F32, 4>(X, Iout, Ia, Ib, S.F[0], S.F[1], S.M, nw, sb, se, part);
+      else {
+        part.band0 = part.band1 = -1;
+        if (v == v0) other();
+      }
+    } else {
+      if (v == v0 && (int)(tid() >> 6) == nw - 1) other();
+      mttkrp_mfma<F32, LAST_Q>(X, Iout, Ia, Ib, S.F[0], S.F[1], S.M, nw, sb, se, part);
     }
-  } else {
-    if ((int)(tid() >> 6) == nw - 1) other();
-    mttkrp_mfma<F32, LAST_Q>(X, Iout, Ia, Ib, S.F[0], S.F[1], S.M, nw, part);
+    mttkrp_mfma_partials<F32>(part, nw, P, S.seg);
+    __syncthreads();
+    mttkrp_mfma_reduce<BS>(Iout, nw, P, S.seg, S.M);
+    __syncthreads();
+    if (V > 1) {
+      double *slot = buf + (int64_t)v * kBMax * kBR;
+      for (int e = tid(); e < Iout * kBR; e += BS)
+        if (piece_touches(sb, se, KS, (e / kBR) >> 5)) slot[e] = S.M[e];
+      __syncthreads();
+    }
   }
-  mttkrp_mfma_partials<F32>(part, nw, P, S.seg);
-  __syncthreads();
-  mttkrp_mfma_reduce<BS>(Iout, nw, P, S.seg, S.M);
-  __syncthreads();
+  if (V > 1) {
+    ++grp.xi;
+    if (grp.G > 1) {
+      group_wait(grp.ctr, grp.G * grp.xi, grp.err);
+    } else {                                 // the block's own pieces: visible, no stale L1 lines
+      __threadfence();
+      __syncthreads();
+      __threadfence();
+    }
+    for (int e = tid(); e < Iout * kBR; e += BS) {
+      const int b = (e / kBR) >> 5;
+      double sum = 0.0;
+      bool any = false;
+      for (int v = 0; v < V; ++v) {
+        const int64_t sb = (int64_t)v * Stot / V, se = (int64_t)(v + 1) * Stot / V;
+        if (piece_touches(sb, se, KS, b)) {
+          const double x = buf[(int64_t)v * kBMax * kBR + e];
+          sum = any ? sum + x : x;
+          any = true;
+        }
+      }
+      S.M[e] = sum;
+    }
+    __syncthreads();
+  }
 }
 
 // NesterovMNLS (NesterovMNLS.hpp:181-234) for the I x R entries of one factor.  The
@@ -447,8 +526,11 @@ __device__ __forceinline__ int nnls_block(int I, int R, BatchLds &S, double *Ag,
 }
 
 // one sweep of one problem
+// Member g of the problem's group: member 0 works on the problem's state and writes its
+// status and log; members g > 0 on private copies of the state (identical values: every
+// member takes the same steps on the same data).
 template <int BS, int RP, bool F32>
-__device__ __forceinline__ void sweep_problem(const BatchArgs &a, int p, int sweep, BatchLds &S, double *P) {
+__device__ __forceinline__ void sweep_problem(const BatchArgs &a, int p, int g, int sweep, BatchLds &S, double *P) {
   constexpr int NW = BS / 64;                  // the last wave solves the eigenproblem first
   const BatchDesc &d = a.desc[p];
   BatchProb &st = a.st[p];
@@ -456,9 +538,13 @@ __device__ __forceinline__ void sweep_problem(const BatchArgs &a, int p, int swe
   const int I0 = d.I[0], I1 = d.I[1], I2 = d.I[2];
   const int R = d.R, t = threadIdx.x;
   const float *X = a.X + d.x_off;
-  double *fac = a.fac + d.f_off * kBR;
+  const bool lead = g == 0;
+  const int64_t nrows_p = d.rowoff[2] + I2;
+  const int64_t hrow = (d.h_off + (int64_t)(g - 1) * nrows_p) * kBR;
+  double *fac = lead ? a.fac + d.f_off * kBR : a.hfac + hrow;
   double *const A0 = fac, *const A1 = fac + (int64_t)d.rowoff[1] * kBR, *const A2 = fac + (int64_t)d.rowoff[2] * kBR;
-  double *gram_g = a.grams + d.g_off;
+  double *gram_g = lead ? a.grams + d.g_off : a.hgrams + d.hg_off + (int64_t)(g - 1) * 3 * kBR * kBR;
+  Group grp{g, d.G, d.V, a.mx + d.mx_off, a.gctr + p, a.err, st.nx};
   for (int e = t; e < 3 * kBR * kBR; e += BS) S.G[e / (kBR * kBR)][e % (kBR * kBR)] = gram_g[e];
   __syncthreads();
 #pragma unroll 1
@@ -482,8 +568,8 @@ __device__ __forceinline__ void sweep_problem(const BatchArgs &a, int p, int swe
     // FP64 matrix cores: the eigen wave (R <= 16: microseconds) then takes 3/4 of an MTTKRP
     // share (1.76 -> 1.69 ms per 512-problem sweep); FP32 ones: the other seven waves split
     // the MTTKRP (the same share made that sweep 1.49 -> 1.53 ms)
-    if constexpr (F32) mttkrp_block<BS, F32, 4>(X + (int64_t)n * a.xcopy, In, Ia, Ib, S, P, NW - 1, eig);
-    else mttkrp_block<BS, F32, 3>(X + (int64_t)n * a.xcopy, In, Ia, Ib, S, P, NW, eig);
+    if constexpr (F32) mttkrp_block<BS, F32, 4>(X + (int64_t)n * a.xcopy, In, Ia, Ib, S, P, NW - 1, grp, eig);
+    else mttkrp_block<BS, F32, 3>(X + (int64_t)n * a.xcopy, In, Ia, Ib, S, P, NW, grp, eig);
     const int it = nnls_block<BS, RP>(In, R, S, n == 0 ? A0 : n == 1 ? A1 : A2, a.d1, a.d2, a.max_inner);
     if (t == 0) S.iters[n] = it;
     gram_block<BS>(S.F[0], In, R, S.G[n]);
@@ -527,7 +613,7 @@ __device__ __forceinline__ void sweep_problem(const BatchArgs &a, int p, int swe
   const bool stop = f / sqrt(d.frob) < a.thr;
   double fa = 0.0;
   bool tried = false, accept = false;
-  double *oldf = a.old + d.f_off * kBR, *accf = a.acc + d.f_off * kBR;
+  double *oldf = lead ? a.old + d.f_off * kBR : a.hold + hrow, *accf = lead ? a.acc + d.f_off * kBR : a.hacc + hrow;
   if (!stop && sweep < a.max_iter && a.acceleration) {
     const int nrows = d.rowoff[2] + I2;
     if (sweep > 1) {
@@ -549,7 +635,7 @@ __device__ __forceinline__ void sweep_problem(const BatchArgs &a, int p, int swe
       __syncthreads();
       gram_block<BS>(S.F[0], I0, R, S.Gh[0]);
       gram_block<BS>(S.F[1], I1, R, S.Gh[1]);
-      mttkrp_block<BS, F32, 4>(X + 2 * a.xcopy, I2, I0, I1, S, P, NW, [] {});
+      mttkrp_block<BS, F32, 4>(X + 2 * a.xcopy, I2, I0, I1, S, P, NW, grp, [] {});
       double ma = 0.0, vg = 0.0;
       for (int e = t; e < I2 * kBR; e += BS) ma = fma(S.M[e], Ah2[e], ma);
       if (t < kBR * kBR) vg = (S.Gh[1][t] * S.Gh[0][t]) * S.Gh[2][t];
@@ -567,7 +653,8 @@ __device__ __forceinline__ void sweep_problem(const BatchArgs &a, int p, int swe
   }
   __syncthreads();
   for (int e = t; e < 3 * kBR * kBR; e += BS) gram_g[e] = S.G[e / (kBR * kBR)][e % (kBR * kBR)];
-  if (t == 0) {
+  if (t == 0 && lead) {
+    st.nx = grp.xi;
     st.f = accept ? fa : f;
     st.f_accel = fa;
     st.done = stop ? 1 : 0;
@@ -593,24 +680,31 @@ __device__ __forceinline__ void sweep_problem(const BatchArgs &a, int p, int swe
   __syncthreads();
 }
 
+// Partial MTTKRP bands (2 x 32 x 16 doubles per wave): in LDS for 512-thread blocks (one
+// block per CU), in the block's slice of global scratch for 256-thread blocks, whose LDS then
+// fits two blocks per CU (two waves per SIMD; the launch bound keeps the registers within it).
 template <int BS, bool F32>
-__global__ __launch_bounds__(BS, BS == 512 ? 2 : 1) void k_batch_sweep(BatchArgs a) {
+__global__ __launch_bounds__(BS, 2) void k_batch_sweep(BatchArgs a) {
   __shared__ BatchLds S;
-  __shared__ double P[(BS / 64) * 2 * 32 * kBR];     // partial MTTKRP bands, per wave
+  __shared__ double Plds[BS == 512 ? (BS / 64) * 2 * 32 * kBR : 1];
+  double *P = BS == 512 ? Plds : a.pscratch + (int64_t)blockIdx.x * kBatchPartDoubles256;
   __shared__ int sp;
   const int sweep = *a.sweep_ctr;
+  // The queue holds the members of each problem's group next to each other (largest work
+  // first): at most one group is incomplete at any time, so a member waiting for the others
+  // waits only for blocks that are free or about to be (the group size never exceeds the grid).
   for (;;) {
     if (threadIdx.x == 0) sp = atomicAdd(&a.queue[sweep], 1);
     __syncthreads();
     const int q = sp;
     __syncthreads();
-    if (q >= a.nprob) break;
-    const int p = a.order[q];
+    if (q >= a.nitems) break;
+    const int item = a.order[q], p = item >> 3, g = item & 7;   // (problem, member)
     switch ((a.desc[p].R + 3) >> 2) {
-      case 1: sweep_problem<BS, 4, F32>(a, p, sweep, S, P); break;
-      case 2: sweep_problem<BS, 8, F32>(a, p, sweep, S, P); break;
-      case 3: sweep_problem<BS, 12, F32>(a, p, sweep, S, P); break;
-      default: sweep_problem<BS, 16, F32>(a, p, sweep, S, P); break;
+      case 1: sweep_problem<BS, 4, F32>(a, p, g, sweep, S, P); break;
+      case 2: sweep_problem<BS, 8, F32>(a, p, g, sweep, S, P); break;
+      case 3: sweep_problem<BS, 12, F32>(a, p, g, sweep, S, P); break;
+      default: sweep_problem<BS, 16, F32>(a, p, g, sweep, S, P); break;
     }
   }
 }
@@ -628,6 +722,20 @@ __global__ void k_batch_permute(const float *__restrict__ X, int I0, int I1, int
       const int64_t i2 = e % I2, r = e / I2, i0 = r % I0, i1 = r / I0;
       X2[e] = X[i0 + I0 * (i1 + I1 * i2)];
     }
+  }
+}
+
+// the problem's state (factor rows, previous rows, Grams) into every helper's copy
+__global__ void k_batch_sync_helpers(BatchArgs a) {
+  const BatchDesc &d = a.desc[blockIdx.x];
+  const int64_t nr = (int64_t)(d.I[0] + d.I[1] + d.I[2]) * kBR, ng = 3 * kBR * kBR;
+  for (int g = 1; g < d.G; ++g) {
+    const int64_t h = (d.h_off + (int64_t)(g - 1) * (d.I[0] + d.I[1] + d.I[2])) * kBR;
+    for (int64_t e = threadIdx.x; e < nr; e += blockDim.x) {
+      a.hfac[h + e] = a.fac[d.f_off * kBR + e];
+      a.hold[h + e] = a.old[d.f_off * kBR + e];
+    }
+    for (int64_t e = threadIdx.x; e < ng; e += blockDim.x) a.hgrams[d.hg_off + (g - 1) * ng + e] = a.grams[d.g_off + e];
   }
 }
 
@@ -677,6 +785,8 @@ void batch_sweep(hipStream_t s, const BatchArgs &a, int block, int grid) {
 }
 
 void batch_next_sweep(hipStream_t s, int *ctr) { k_batch_next<<<1, 64, 0, s>>>(ctr); }
+
+void batch_sync_helpers(hipStream_t s, const BatchArgs &a) { k_batch_sync_helpers<<<a.nprob, 256, 0, s>>>(a); }
 
 void batch_permute(hipStream_t s, const float *X, int I0, int I1, int I2, float *X1, float *X2) {
   const int64_t n = (int64_t)I0 * I1 * I2;

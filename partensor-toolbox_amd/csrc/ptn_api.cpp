@@ -1864,8 +1864,8 @@ ptn_status ptn_cpd_batch(ptn_handle h, const double *init_packed, const ptn_opti
     REQUIRE(o.accel_fail >= 0, PTN_ERR_INVALID, "accel_fail must be >= 0");
     for (int k = 0; k < 3; ++k)
       REQUIRE(o.constraints[k] == PTN_NONNEGATIVITY, PTN_ERR_UNSUPPORTED, "only the nonnegativity constraint is built");
-    REQUIRE((block == 0 && grid == 0) || ((block == 256 || block == 512) && grid >= 1 && grid <= (1 << 16)),
-            PTN_ERR_INVALID, "block/grid: both 0 (autotune) or block 256 / 512 with grid in [1, 65536]");
+    REQUIRE((block == 0 && grid == 0) || ((block == 256 || block == 512) && grid >= 1 && grid <= 8192),
+            PTN_ERR_INVALID, "block/grid: both 0 (autotune) or block 256 / 512 with grid in [1, 8192]");
     REQUIRE(o.arith == PTN_ARITH_F64 || o.arith == PTN_ARITH_F32, PTN_ERR_UNSUPPORTED,
             "batch MTTKRP arithmetic: PTN_ARITH_F64 (FP64 matrix cores) or PTN_ARITH_F32 (FP32 matrix cores)");
     const BatchSet &b = *h->batch;
@@ -1890,14 +1890,45 @@ ptn_status ptn_cpd_batch(ptn_handle h, const double *init_packed, const ptn_opti
       q.accel_coeff = o.accel_coeff;
       q.accel_fail = o.accel_fail;
     }
-    // largest work first (tensor elements times the padded rank), ties by index
-    std::vector<int> order(P);
-    std::iota(order.begin(), order.end(), 0);
+    // Work of a problem: tensor elements times the padded rank.  A problem with more than
+    // kSplitWork times the batch's mean work has its MTTKRP sums cut in
+    // V = ceil(work / (kSplitWork mean)) pieces (at most 8), so that up to V blocks share it
+    // (BatchDesc::V); the pieces, and with them every result, depend on the batch only.  The
+    // other members repeat the solves, so splitting pays only for the largest problems: on the
+    // 512-problem batch (FP32 matrix cores, 512 x 256 launch) a factor of 4 gave 1.43 ms per
+    // sweep, 2.2 to 6 gave 1.41-1.46 ms, 1.5 gave 1.76 ms and no split 1.50 ms.
+    constexpr double kSplitWork = 4.0;
+    std::vector<BatchDesc> dv(b.desc);
     auto work = [&](int p) {
       const BatchDesc &d = b.desc[p];
-      return batch_elems(d) * ((d.R + 3) / 4 * 4 + 4);
+      return (double)batch_elems(d) * ((d.R + 3) / 4 * 4 + 4);
     };
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work(x) > work(y); });
+    double mean = 0.0;
+    for (int p = 0; p < P; ++p) mean += work(p) / P;
+    int64_t hrows = 0, hgr = 0, mxs = 0;
+    for (int p = 0; p < P; ++p) {
+      BatchDesc &d = dv[p];
+      d.V = std::max(1, std::min(kBatchMaxMembers, (int)std::ceil(work(p) / (kSplitWork * mean))));
+      d.G = 1;
+      d.h_off = hrows;
+      d.hg_off = hgr;
+      d.mx_off = mxs;
+      hrows += (int64_t)(d.V - 1) * (d.I[0] + d.I[1] + d.I[2]);
+      hgr += (int64_t)(d.V - 1) * 3 * kBatchBR * kBatchBR;
+      if (d.V > 1) mxs += (int64_t)2 * d.V * kBatchMaxDim * kBatchBR;
+    }
+    // Queue of one launch shape: each problem runs on G = min(V, grid) blocks (members next
+    // to each other in the queue), problems ordered by the work of one member, largest first
+    std::vector<int> order;
+    auto shape_queue = [&](int grid_blocks) {
+      std::vector<int> pr(P);
+      std::iota(pr.begin(), pr.end(), 0);
+      for (int p = 0; p < P; ++p) dv[p].G = std::min(dv[p].V, grid_blocks);
+      std::stable_sort(pr.begin(), pr.end(), [&](int x, int y) { return work(x) / dv[x].G > work(y) / dv[y].G; });
+      order.clear();
+      for (int p : pr)
+        for (int g = 0; g < dv[p].G; ++g) order.push_back(8 * p + g);
+    };
     // (sweep + 1)^(1 / coeff) for every sweep and every coefficient the bookkeeping can reach
     const int stride = o.accel_coeff + o.max_iter / 5 + 3;
     std::vector<double> pw((int64_t)(o.max_iter + 1) * stride, 0.0);
@@ -1908,13 +1939,17 @@ ptn_status ptn_cpd_batch(ptn_handle h, const double *init_packed, const ptn_opti
     DevBuf<BatchProb> st(h, P), st_snap(h, P);
     DevBuf<double> fac(h, nr), old(h, nr), acc(h, nr), grams(h, (size_t)P * 3 * kBatchBR * kBatchBR);
     DevBuf<double> fac_snap(h, nr), old_snap(h, nr), grams_snap(h, grams.n), powt(h, pw.size());
-    DevBuf<int> ord(h, P), queue(h, o.max_iter + 2), ctr(h, 1);
+    DevBuf<double> hfac(h, hrows * kBatchBR), hold(h, hrows * kBatchBR), hacc(h, hrows * kBatchBR), hgrams(h, hgr),
+        mx(h, mxs);
+    DevBuf<int> ord(h, (size_t)P * kBatchMaxMembers), queue(h, o.max_iter + 2), ctr(h, 1), gctr(h, P), gerr(h, 1);
+    int cu = 0;
+    HIPCHK(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, h->device));
+    // partial bands of 256-thread blocks: one slice per block of the largest grid launched
+    DevBuf<double> pscr(h, (size_t)std::max(block == 0 ? 2 * cu : grid, 1) * kBatchPartDoubles256);
     DevBuf<BatchLog> lg(h, log ? (size_t)P * o.max_iter : 1);
     const int one = 1;
-    HIPCHK(hipMemcpyAsync(desc.p, b.desc.data(), P * sizeof(BatchDesc), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(st.p, st0.data(), P * sizeof(BatchProb), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(fac.p, rows.data(), nr * sizeof(double), hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(ord.p, order.data(), P * sizeof(int), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(powt.p, pw.data(), pw.size() * sizeof(double), hipMemcpyHostToDevice, s));
 
     BatchArgs a{};
@@ -1941,13 +1976,36 @@ ptn_status ptn_cpd_batch(ptn_handle h, const double *init_packed, const ptn_opti
     a.log = nullptr;
     a.arith_f32 = o.arith == PTN_ARITH_F32 ? 1 : 0;
     a.xcopy = b.nx;
+    a.hfac = hfac.p;
+    a.hold = hold.p;
+    a.hacc = hacc.p;
+    a.hgrams = hgrams.p;
+    a.mx = mx.p;
+    a.gctr = gctr.p;
+    a.err = gerr.p;
+    a.pscratch = pscr.p;
+    // the queue of a launch shape; BatchDesc::G follows it (hipMemcpyAsync from pageable
+    // memory returns once the source has been read, so dv and order may change afterwards)
+    auto set_shape = [&](int grid_blocks) {
+      shape_queue(grid_blocks);
+      HIPCHK(hipMemcpyAsync(desc.p, dv.data(), P * sizeof(BatchDesc), hipMemcpyHostToDevice, s));
+      HIPCHK(hipMemcpyAsync(ord.p, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice, s));
+      HIPCHK(hipStreamSynchronize(s));
+      a.nitems = (int)order.size();
+    };
+    set_shape(1);
     batch_init(s, a);                                   // ||X_p||^2 and the initial Grams
+    // the host copy takes the norms, so that later uploads of dv keep them
+    HIPCHK(hipMemcpyAsync(dv.data(), desc.p, P * sizeof(BatchDesc), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
 
-    int cu = 0;
-    HIPCHK(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, h->device));
+    // a run from the current state: sweep counters, group arrivals, the helpers' copies
     auto restart = [&] {
       HIPCHK(hipMemsetAsync(queue.p, 0, queue.n * sizeof(int), s));
+      HIPCHK(hipMemsetAsync(gctr.p, 0, gctr.n * sizeof(int), s));
+      HIPCHK(hipMemsetAsync(gerr.p, 0, sizeof(int), s));
       HIPCHK(hipMemcpyAsync(ctr.p, &one, sizeof(int), hipMemcpyHostToDevice, s));
+      batch_sync_helpers(s, a);
     };
     double tune_s = 0.0;
     const bool tune = block == 0;
@@ -1971,6 +2029,7 @@ ptn_status ptn_cpd_batch(ptn_handle h, const double *init_packed, const ptn_opti
       double best = 1e300;
       TimingEvents ev(h);
       for (const auto &c : cand) {
+        set_shape(c[1]);
         restore();
         HIPCHK(hipEventRecord(ev.e[0], s));
         for (int k = 0; k < tsweeps; ++k) {
@@ -1986,10 +2045,12 @@ ptn_status ptn_cpd_batch(ptn_handle h, const double *init_packed, const ptn_opti
           grid = c[1];
         }
       }
+      set_shape(grid);
       restore();
       HIPCHK(hipStreamSynchronize(s));
       tune_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     } else {
+      set_shape(grid);
       restart();
     }
 
@@ -2016,11 +2077,14 @@ ptn_status ptn_cpd_batch(ptn_handle h, const double *init_packed, const ptn_opti
 
     std::vector<BatchProb> stf(P);
     std::vector<BatchDesc> df(P);
+    int gfail = 0;
+    HIPCHK(hipMemcpyAsync(&gfail, gerr.p, sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(stf.data(), st.p, P * sizeof(BatchProb), hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(df.data(), desc.p, P * sizeof(BatchDesc), hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(rows.data(), fac.p, nr * sizeof(double), hipMemcpyDeviceToHost, s));
     if (log) HIPCHK(hipMemcpyAsync(log, lg.p, lg.n * sizeof(BatchLog), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
+    REQUIRE(!gfail, PTN_ERR_HIP, "a batch group's exchange wait ran out (members not co-resident?)");
     int64_t dst = 0;
     for (int p = 0; p < P; ++p) {
       const BatchDesc &d = b.desc[p];
@@ -2047,6 +2111,12 @@ ptn_status ptn_cpd_batch(ptn_handle h, const double *init_packed, const ptn_opti
       rep->seconds = ms * 1e-3;
       rep->tune_seconds = tune_s;
       rep->sweeps = o.max_iter;
+      rep->split_problems = 0;
+      rep->max_pieces = 1;
+      for (const BatchDesc &d : dv) {
+        rep->split_problems += d.V > 1;
+        rep->max_pieces = std::max(rep->max_pieces, d.V);
+      }
     }
   });
 }

@@ -39,13 +39,23 @@ struct NnlsParams {
 // ---- batched small CPDs (batch_kernels.hip) ----
 // One order-3 problem of a batch: sizes, rank (<= 16), where its tensor, factor rows
 // (rows of 16 doubles, factors 0, 1, 2 back to back) and Grams (3 x 16 x 16) live.
+// A problem's MTTKRPs are summed in V "virtual members" (V = 1: one block's waves, as
+// for small problems; V > 1: the (band, k-step) space cut in V contiguous pieces, each
+// summed by a block's waves, the pieces added in order), so its result does not depend
+// on how many blocks run it: G blocks (members) each take V / G pieces, exchange them
+// through global memory and then all run the rest of the sweep on identical data; member
+// 0 keeps the problem's state, members 1 .. G-1 private copies of it (h_off, hg_off).
 struct BatchDesc {
   int I[3];
   int R;
   int rowoff[3];
-  int pad;
+  int V;                       // virtual members of the MTTKRP sums
   int64_t x_off, f_off, g_off;
   double frob;                 // ||X||^2, set by batch_init
+  int G;                       // blocks running the problem (1 .. V)
+  int pad;
+  int64_t h_off, hg_off;       // the helpers' factor rows / Grams (helper g at (g - 1) x size)
+  int64_t mx_off;              // exchange area: [2 parities][V][128 x 16] doubles
 };
 // ALS state of one problem, kept on the device between sweeps.
 struct BatchProb {
@@ -55,7 +65,7 @@ struct BatchProb {
   int sweeps;                  // sweeps completed
   int tried, accepted;         // line search of the last sweep
   int nnls_iters[3];
-  int pad;
+  int nx;                      // group exchanges done (members wait for G x nx arrivals)
 };
 // One record per problem and sweep (layout of ptn_batch_log in the C ABI).
 struct BatchLog {
@@ -81,7 +91,17 @@ struct BatchArgs {
   BatchLog *log;               // [nprob][max_iter] (optional)
   int arith_f32;               // 1: MTTKRP on the FP32 matrix cores (FP64 totals), 0: FP64 matrix cores
   int64_t xcopy;               // X + n xcopy: the tensors with mode n fastest ((i1, i0, i2), (i2, i0, i1) for n = 1, 2)
+  int nitems;                  // queue entries: one per (problem, member), order[q] = 8 p + member
+  double *hfac, *hold, *hacc, *hgrams;   // helpers' private state (BatchDesc::h_off, hg_off)
+  double *mx;                  // exchange areas (BatchDesc::mx_off)
+  int *gctr;                   // per problem: arrivals at the group's exchanges
+  int *err;                    // set when a group wait runs out (the run is then reported failed)
+  double *pscratch;            // 256-thread blocks: partial MTTKRP bands, kBatchPartDoubles256 per block
 };
+constexpr int kBatchMaxMembers = 8;
+constexpr int64_t kBatchPartDoubles256 = 4 * 2 * 32 * 16;
+// copies every problem's state to its helpers' private copies (after set-up or a restore)
+void batch_sync_helpers(hipStream_t s, const BatchArgs &a);
 size_t batch_lds_bytes();
 void batch_init(hipStream_t s, const BatchArgs &a);
 void batch_sweep(hipStream_t s, const BatchArgs &a, int block, int grid);

@@ -73,7 +73,7 @@ class BatchLog(ctypes.Structure):
 
 class BatchReport(ctypes.Structure):
     _fields_ = [("block", _i), ("grid", _i), ("tuned", _i), ("sweeps", _i), ("seconds", _d),
-                ("tune_seconds", _d)]
+                ("tune_seconds", _d), ("split_problems", _i), ("max_pieces", _i)]
 
 
 class PtnError(RuntimeError):
@@ -387,7 +387,8 @@ class Handle:
                                "stopped": r.stopped})
                 log.append(lp)
         report = {"block": rep.block, "grid": rep.grid, "tuned": rep.tuned, "sweeps": rep.sweeps,
-                  "seconds": rep.seconds, "tune_seconds": rep.tune_seconds}
+                  "seconds": rep.seconds, "tune_seconds": rep.tune_seconds, "split_problems": rep.split_problems,
+                  "max_pieces": rep.max_pieces}
         return self._unpack_batch(out), status, log, report
 
     # -- single operations --

@@ -110,6 +110,39 @@ def test_batch_parity_vs_oracle(h):
     say("over the batch vs oracle: max factor rel diff %.2e, max |f^2 - f_o^2| / ||X||^2 %.2e" % tuple(worst))
 
 
+def test_batch_split_problems_independent_of_blocks(h):
+    """A problem with far more work than the batch's mean has its MTTKRP sums cut in pieces
+    (8 here) that up to 8 blocks share, exchanging them at a group barrier: on 1, 3 and 40
+    blocks (1, 3 and 8 members) the factors are bitwise equal, and the run matches the
+    oracle like an unsplit problem (both arithmetics)."""
+    probs = [((120, 110, 100), 12)] + [((6 + p % 5, 5 + p % 3, 7), 1 + p % 4) for p in range(40)]
+    say("split batch:", probs[0], "+", len(probs) - 1, "small problems")
+    Xs = _tensors(probs, seed0=500)
+    init = [o.gen_factors(list(d), R, seed=600 + p) for p, (d, R) in enumerate(probs)]
+    h.batch_set(Xs, [R for _, R in probs])
+    kw = dict(max_iter=10, threshold_error=0.0)
+    for arith in (P.ARITH_F64, P.ARITH_F32):
+        runs = []
+        for grid in (1, 3, 40):
+            F, st, lg, rep = h.cpd_batch(init, block=512, grid=grid, arith=arith, **kw)
+            say("arith", arith, "grid", grid, "report", rep)
+            assert rep["split_problems"] == 1 and rep["max_pieces"] == 8
+            runs.append((F, st, lg))
+        for F, st, _ in runs[1:]:
+            assert [s_["f_value"] for s_ in st] == [s_["f_value"] for s_ in runs[0][1]]
+            for Fa, Fb in zip(F, runs[0][0]):
+                for a, b in zip(Fa, Fb):
+                    assert np.array_equal(a, b)
+        F, st, lg = runs[-1]
+        if arith == P.ARITH_F64:
+            Fo, sto, logo, tol = oracle_problem(Xs[0], probs[0][1], init[0], lg[0], **kw)
+            err, cgap = _check_problem(0, F[0], st[0], lg[0], Fo, sto, logo, tol)
+            say("split problem vs oracle: factors %.2e (tolerance %.1e), cost %.2e" % (err, tol, cgap))
+        else:
+            Fo, sto, _ = o.cpd(Xs[0], probs[0][1], init[0], cost_form=o.COST_GRAM, **kw)
+            assert st[0]["rel_cost"] == pytest.approx(sto["rel_cost"], abs=2e-6)
+
+
 def test_batch_fp32_matrix_cores_vs_fp64(h):
     """arith = F32 runs the batch MTTKRPs on the FP32 matrix cores (FP32 products of the FP32
     tensor and the FP32-rounded Khatri-Rao values, FP64 totals every 12 k-steps); every
