@@ -22,6 +22,7 @@
 namespace ptn {
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
 
 namespace {
 
@@ -46,7 +47,7 @@ __device__ __forceinline__ double ld(const TX *p) { return (double)__builtin_non
 // interleaved k-steps of 4 columns and are summed through LDS at the end.
 // ---------------------------------------------------------------------------
 template <typename TX, int NT>
-__global__ __launch_bounds__(256) void k_gemm_cm_f64(const TX *__restrict__ X, int64_t Mrows, int64_t Ktot,
+__global__ __launch_bounds__(256, 2) void k_gemm_cm_f64(const TX *__restrict__ X, int64_t Mrows, int64_t Ktot,
                                                      KrArgs w, int ldr, int64_t kchunk, double *__restrict__ out) {
   constexpr int RP = NT * 16;
   __shared__ double red[64 * RP];
@@ -62,9 +63,16 @@ __global__ __launch_bounds__(256) void k_gemm_cm_f64(const TX *__restrict__ X, i
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[s][t] = f64x4{0.0, 0.0, 0.0, 0.0};
 
-  bool rv[4];
+  // Every load is unconditional, so that loads stay in flight across the MFMAs: a row past
+  // Mrows reads row Mrows - 1 (its sums are never stored); a column past the chunk reads
+  // the chunk's first column and a factor row 0, and both operands of that column are
+  // replaced by zero where the MFMAs use them.
+  int64_t roff[4];
 #pragma unroll
-  for (int s = 0; s < 4; ++s) rv[s] = mb + 16 * s + f < Mrows;
+  for (int s = 0; s < 4; ++s) {
+    const int64_t m = mb + 16 * s + f;
+    roff[s] = m < Mrows ? m : Mrows - 1;
+  }
 
   // digits of this lane's first column kk = kbeg + 4*wave + g (lowest mode fastest)
   int64_t kk = kbeg + 4 * wave + g;
@@ -73,33 +81,88 @@ __global__ __launch_bounds__(256) void k_gemm_cm_f64(const TX *__restrict__ X, i
     uint64_t rem = (uint64_t)kk;
     for (int j = 0; j < w.nm; ++j) { dig[j] = (uint32_t)(rem % w.dim[j]); rem /= w.dim[j]; }
   }
-  for (; kk - g < kend; kk += 16) {
-    const bool kv = kk < kend;
-    double a[4];
-    const TX *xp = X + mb + f + Mrows * kk;
-#pragma unroll
-    for (int s = 0; s < 4; ++s) a[s] = (kv && rv[s]) ? ld(xp + 16 * s) : 0.0;
-    double b[NT];
+  // The Khatri-Rao values of the lane's current column as two factors, p (the product of
+  // all but the last mode's rows, left to right) and q (the last mode's row; 1 for one
+  // mode), so that p * q is the left-to-right product and the last multiply can wait
+  // until the MFMAs need it; then the odometer advanced by 16.
+  auto kr_next = [&](bool kv, double (&p)[NT], double (&q)[NT]) {
+    const int last = w.nm - 1;
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-      double p = 0.0;
-      if (kv) {
-        p = w.fac[0][(int64_t)dig[0] * ldr + 16 * t + f];
-        for (int j = 1; j < w.nm; ++j) p *= w.fac[j][(int64_t)dig[j] * ldr + 16 * t + f];
-      }
-      b[t] = p;
+      double pv = w.fac[0][(int64_t)(kv ? dig[0] : 0u) * ldr + 16 * t + f];
+      for (int j = 1; j < last; ++j) pv *= w.fac[j][(int64_t)(kv ? dig[j] : 0u) * ldr + 16 * t + f];
+      p[t] = pv;
+      q[t] = last > 0 ? w.fac[last][(int64_t)(kv ? dig[last] : 0u) * ldr + 16 * t + f] : 1.0;
     }
-#pragma unroll
-    for (int s = 0; s < 4; ++s)
-#pragma unroll
-      for (int t = 0; t < NT; ++t) acc[s][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b[t], acc[s][t], 0, 0, 0);
-    // advance the odometer by 16 columns
     dig[0] += 16;
     for (int j = 0; j + 1 < w.nm && dig[j] >= w.dim[j]; ++j) {
-      uint32_t q = dig[j] / w.dim[j];
-      dig[j] -= q * w.dim[j];
-      dig[j + 1] += q;
+      uint32_t c = dig[j] / w.dim[j];
+      dig[j] -= c * w.dim[j];
+      dig[j + 1] += c;
     }
+  };
+  // one step of 16 columns (4 per wave): loads, then (apply) the zeroing and the MFMAs
+  auto load_step = [&](bool &kv, double (&a)[4], double (&p)[NT], double (&q)[NT]) {
+    kv = kk < kend;
+    kr_next(kv, p, q);
+    const TX *xp = X + Mrows * (kv ? kk : kbeg);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) a[s] = ld(xp + roff[s]);
+    kk += 16;
+  };
+  auto mma_step = [&](bool kv, const double (&a)[4], const double (&p)[NT], const double (&q)[NT]) {
+    double b[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) b[t] = kv ? p[t] * q[t] : 0.0;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const double av = kv ? a[s] : 0.0;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) acc[s][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b[t], acc[s][t], 0, 0, 0);
+    }
+  };
+  // Groups of KU steps, software-pipelined over two register buffers: group G+1's loads
+  // are issued before group G's MFMAs, with no branch between a load and its use inside
+  // the loop.  The MFMAs run in step order (the same sums as one step at a time).
+  constexpr int KU = NT == 1 ? 4 : NT == 2 ? 2 : 1;
+  struct Buf {
+    bool kv[KU];
+    double a[KU][4], p[KU][NT], q[KU][NT];
+  };
+  auto load_group = [&](Buf &B) {
+#pragma unroll
+    for (int u = 0; u < KU; ++u) load_step(B.kv[u], B.a[u], B.p[u], B.q[u]);
+  };
+  auto mma_group = [&](const Buf &B) {
+#pragma unroll
+    for (int u = 0; u < KU; ++u) mma_step(B.kv[u], B.a[u], B.p[u], B.q[u]);
+  };
+  const int64_t kw = kk - g;                                 // the wave's first column
+  const int64_t nsteps = kend > kw ? (kend - kw + 15) / 16 : 0;
+  const int64_t ngroups = nsteps / KU;
+  if (ngroups > 0) {
+    Buf b0, b1;
+    load_group(b0);
+    int64_t G = 0;
+    for (; G + 2 < ngroups; G += 2) {
+      load_group(b1);
+      mma_group(b0);
+      load_group(b0);
+      mma_group(b1);
+    }
+    if (G + 1 < ngroups) {
+      load_group(b1);
+      mma_group(b0);
+      mma_group(b1);
+    } else {
+      mma_group(b0);
+    }
+  }
+  for (int64_t j = ngroups * KU; j < nsteps; ++j) {
+    bool kv;
+    double a[4], p[NT], q[NT];
+    load_step(kv, a, p, q);
+    mma_step(kv, a, p, q);
   }
 
   // sum the 4 waves' tiles in LDS, in wave order
@@ -129,8 +192,8 @@ __global__ __launch_bounds__(256) void k_gemm_cm_f64(const TX *__restrict__ X, i
 // Within a 16-wide k-block, lane group g supplies a = kb + 4g + t at MFMA step t,
 // so each lane reads 4 consecutive elements of its fiber.
 // ---------------------------------------------------------------------------
-template <typename TX, int NT>
-__global__ __launch_bounds__(256) void k_gemm_ck_f64(const TX *__restrict__ X, int64_t L, int64_t C, KrArgs w,
+template <typename TX, int NT, bool VEC>
+__global__ __launch_bounds__(256, NT <= 2 ? 2 : 1) void k_gemm_ck_f64(const TX *__restrict__ X, int64_t L, int64_t C, KrArgs w,
                                                      int ldr, double *__restrict__ T) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int g = lane >> 4, f = lane & 15;
@@ -151,7 +214,78 @@ __global__ __launch_bounds__(256) void k_gemm_ck_f64(const TX *__restrict__ X, i
     cv[s] = c < C;
     xp[s] = X + L * (cv[s] ? c : 0);
   }
-  for (int64_t kb = 0; kb < L; kb += 16) {
+  // the Khatri-Rao row of the contracted modes at a (lowest mode fastest), NT fragments
+  auto kr_row = [&](int64_t a, bool av, double (&b)[NT]) {
+    const double *brow[kMaxOrder];
+    if (av) {
+      uint32_t rem = (uint32_t)a;                 // a < L: the last digit is what remains
+      for (int j = 0; j + 1 < w.nm; ++j) {
+        uint32_t d = rem % w.dim[j];
+        rem /= w.dim[j];
+        brow[j] = w.fac[j] + (int64_t)d * ldr;
+      }
+      brow[w.nm - 1] = w.fac[w.nm - 1] + (int64_t)rem * ldr;
+    }
+#pragma unroll
+    for (int u = 0; u < NT; ++u) {
+      double p = 0.0;
+      if (av) {
+        p = brow[0][16 * u + f];
+        for (int j = 1; j < w.nm; ++j) p *= brow[j][16 * u + f];
+      }
+      b[u] = p;
+    }
+  };
+  auto mma16 = [&](const double (&x)[4][4], const double (&b)[4][NT]) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int u = 0; u < NT; ++u)
+          acc[s][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[s][t], b[t][u], acc[s][u], 0, 0, 0);
+  };
+  int64_t kb = 0;
+  if constexpr (VEC) {
+    // L % 4 == 0 and FP64 data: a lane's four elements a = kb + 4g .. kb + 4g + 3 of a fiber
+    // are one aligned 32-byte chunk (two 16-byte loads).  Full 16-wide blocks are
+    // software-pipelined over two register buffers: block j+1's Khatri-Rao rows and tensor
+    // loads are issued before block j's MFMAs.
+    const int64_t nblk = L / 16;
+    auto load_blk = [&](double (&x)[4][4], double (&b)[4][NT]) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t) kr_row(kb + 4 * g + t, true, b[t]);
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const f64x2 *q = (const f64x2 *)(xp[s] + kb + 4 * g);
+        const f64x2 lo = __builtin_nontemporal_load(q), hi = __builtin_nontemporal_load(q + 1);
+        x[s][0] = lo[0];                            // a fiber past C reads fiber 0; its
+        x[s][1] = lo[1];                            // sums are never stored
+        x[s][2] = hi[0];
+        x[s][3] = hi[1];
+      }
+      kb += 16;
+    };
+    if (nblk > 0) {
+      double x0[4][4], b0[4][NT], x1[4][4], b1[4][NT];
+      load_blk(x0, b0);
+      int64_t j = 0;
+      for (; j + 2 < nblk; j += 2) {               // no branch between a load and its use
+        load_blk(x1, b1);
+        mma16(x0, b0);
+        load_blk(x0, b0);
+        mma16(x1, b1);
+      }
+      if (j + 1 < nblk) {
+        load_blk(x1, b1);
+        mma16(x0, b0);
+        mma16(x1, b1);
+      } else {
+        mma16(x0, b0);
+      }
+    }
+  }
+  for (; kb < L; kb += 16) {
     double x[4][4];
     double b[4][NT];
 #pragma unroll
@@ -160,33 +294,9 @@ __global__ __launch_bounds__(256) void k_gemm_ck_f64(const TX *__restrict__ X, i
       const bool av = a < L;
 #pragma unroll
       for (int s = 0; s < 4; ++s) x[s][t] = (av && cv[s]) ? ld(xp[s] + a) : 0.0;
-      // Khatri-Rao row of the contracted modes at a (lowest mode fastest)
-      const double *brow[kMaxOrder];
-      if (av) {
-        uint32_t rem = (uint32_t)a;
-        for (int j = 0; j < w.nm; ++j) {
-          uint32_t d = rem % w.dim[j];
-          rem /= w.dim[j];
-          brow[j] = w.fac[j] + (int64_t)d * ldr;
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < NT; ++u) {
-        double p = 0.0;
-        if (av) {
-          p = brow[0][16 * u + f];
-          for (int j = 1; j < w.nm; ++j) p *= brow[j][16 * u + f];
-        }
-        b[t][u] = p;
-      }
+      kr_row(a, av, b[t]);
     }
-#pragma unroll
-    for (int t = 0; t < 4; ++t)
-#pragma unroll
-      for (int s = 0; s < 4; ++s)
-#pragma unroll
-        for (int u = 0; u < NT; ++u)
-          acc[s][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[s][t], b[t][u], acc[s][u], 0, 0, 0);
+    mma16(x, b);
   }
 #pragma unroll
   for (int s = 0; s < 4; ++s)
@@ -412,11 +522,13 @@ void gemm_ck_f64(hipStream_t s, const double *x64, const float *x32, int64_t L, 
   KrArgs a = to_args(w);
   dim3 grid((unsigned)((C + 255) / 256));
   const int NT = ldr / 16;
-#define PTN_CK(NTV)                                                                    \
-  if (NT == NTV) {                                                                     \
-    if (x64) k_gemm_ck_f64<double, NTV><<<grid, 256, 0, s>>>(x64, L, C, a, ldr, T);   \
-    else     k_gemm_ck_f64<float, NTV><<<grid, 256, 0, s>>>(x32, L, C, a, ldr, T);    \
-    return;                                                                            \
+  const bool vec = x64 && L % 4 == 0;         // aligned 32-byte chunks of every fiber
+#define PTN_CK(NTV)                                                                          \
+  if (NT == NTV) {                                                                           \
+    if (vec) k_gemm_ck_f64<double, NTV, true><<<grid, 256, 0, s>>>(x64, L, C, a, ldr, T);     \
+    else if (x64) k_gemm_ck_f64<double, NTV, false><<<grid, 256, 0, s>>>(x64, L, C, a, ldr, T); \
+    else     k_gemm_ck_f64<float, NTV, false><<<grid, 256, 0, s>>>(x32, L, C, a, ldr, T);     \
+    return;                                                                                  \
   }
   PTN_CK(1) PTN_CK(2) PTN_CK(3) PTN_CK(4)
 #undef PTN_CK
