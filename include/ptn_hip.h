@@ -65,6 +65,8 @@ typedef struct {
   int      warm_start;          /* 1: start each NNLS from the handle's network where it passes the KKT test */
   double   warm_kkt_tol;        /* a row takes the network's start if its relative KKT residual
                                    max|min(L a, a Zt - Wt)| / max|Wt| is <= this (default 1e-6) */
+  int      warm_refine;         /* Nesterov steps run from every proposal before that test (0 .. max_inner,
+                                   default 0); they are counted in nnls_iters */
   int      mlp_harvest;         /* 1: append every NNLS (inputs, solution) pair to the handle's training set */
   int      time_passes;         /* 1: HIP events around the two tensor passes of every sweep (t_pass_ms) */
 } ptn_options;
@@ -94,6 +96,7 @@ typedef struct {
   double t_pass_ms[2]; /* time_passes: device time of the tensor passes launched in this sweep, [0] the
                           left-node pass (X x_2 A_2, or its pair with the line search), [1] the right-node
                           pass (dimension tree, FP32 contractions, order 3); 0 where none ran */
+  int    warm_steps[PTN_MAX_ORDER]; /* of nnls_iters, the steps run from the proposals (warm_refine) */
 } ptn_sweep_log;
 
 /* ---- handle ---- */

@@ -197,6 +197,7 @@ __global__ void k_params_no_nnls(NnlsParams *p) {
     p->iters = 0;
     p->fallback = 0;
     p->warm_rows = 0;
+    p->warm_steps = 0;
   }
 }
 

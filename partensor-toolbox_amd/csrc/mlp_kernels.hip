@@ -123,13 +123,16 @@ __global__ __launch_bounds__(256) void k_mlp_features(int64_t I, int R, int ldr,
 // max|wt| / L:  r(a) = max_c |min(L a_c, g_c)| / max_c |wt_c|.
 // The row starts from the network's proposal a_p = max(y, 0) * scale when y is finite and
 // r(a_p) <= kkt_tol; otherwise from a_prev (the fallback, which is exactly the reference's
-// start).  S receives the start rows; count the rows that took a_p.
+// start).  With C, the proposal is C's row instead (max(C, 0): the proposals after some
+// Nesterov steps, whose count, prm->iters, is kept in prm->warm_steps).  S receives the
+// start rows (S may be C); count, when given, the rows that took the proposal.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_warm_select(int64_t I, int R, int ldr, const double *__restrict__ V,
-                                                     const NnlsParams *__restrict__ prm, const double *__restrict__ M,
+                                                     NnlsParams *prm, const double *__restrict__ M,
                                                      const double *__restrict__ A, const float *__restrict__ Y,
-                                                     const float *__restrict__ scale, double kkt_tol,
-                                                     double *__restrict__ S, int *count) {
+                                                     const float *__restrict__ scale, const double *C,
+                                                     double kkt_tol, double *S, int *count) {
+  if (C && blockIdx.x == 0 && threadIdx.x == 0) prm->warm_steps = prm->iters;
   __shared__ double buf[4][kMlpR];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t row = (int64_t)blockIdx.x * 4 + wave;
@@ -137,9 +140,17 @@ __global__ __launch_bounds__(256) void k_warm_select(int64_t I, int R, int ldr, 
   const bool on = lane < R;
   const double lambda = prm->lambda, L = prm->L;
   const double ap = on ? A[row * ldr + lane] : 0.0;
-  const float y = on ? Y[row * kMlpR + lane] : 0.f;
-  const bool fin = !on || isfinite(y);
-  const double aw = (on && y > 0.f) ? (double)y * (double)scale[row] : 0.0;
+  bool fin;
+  double aw;
+  if (C) {
+    const double c = on ? C[row * ldr + lane] : 0.0;
+    fin = !on || isfinite(c);
+    aw = c > 0.0 ? c : 0.0;
+  } else {
+    const float y = on ? Y[row * kMlpR + lane] : 0.f;
+    fin = !on || isfinite(y);
+    aw = (on && y > 0.f) ? (double)y * (double)scale[row] : 0.0;
+  }
   buf[wave][lane] = aw;
   __builtin_amdgcn_wave_barrier();
   double gw = 0.0;
@@ -162,7 +173,7 @@ __global__ __launch_bounds__(256) void k_warm_select(int64_t I, int R, int ldr, 
   const double den = sw > 0.0 ? sw : 1.0;
   const bool take = all_fin && rw / den <= kkt_tol;
   if (lane < ldr) S[row * ldr + lane] = take ? (on ? aw : 0.0) : (lane < R ? ap : A[row * ldr + lane]);
-  if (take && lane == 0) atomicAdd(count, 1);
+  if (count && take && lane == 0) atomicAdd(count, 1);
 }
 
 // loss = sum over valid entries (c < rank[row]) of (Y - T)^2 / count; dY = 2 (Y - T) / count
@@ -265,9 +276,10 @@ void mlp_features(hipStream_t s, int64_t I, int R, int ldr, const double *V, con
   k_mlp_features<<<(unsigned)((I + 3) / 4), 256, 0, s>>>(I, R, ldr, V, M, A, Anew, (__bf16 *)X, T, rank, scale);
 }
 
-void warm_select(hipStream_t s, int64_t I, int R, int ldr, const double *V, const NnlsParams *p, const double *M,
-                 const double *A, const float *Y, const float *scale, double kkt_tol, double *S, int *count) {
-  k_warm_select<<<(unsigned)((I + 3) / 4), 256, 0, s>>>(I, R, ldr, V, p, M, A, Y, scale, kkt_tol, S, count);
+void warm_select(hipStream_t s, int64_t I, int R, int ldr, const double *V, NnlsParams *p, const double *M,
+                 const double *A, const float *Y, const float *scale, const double *C, double kkt_tol, double *S,
+                 int *count) {
+  k_warm_select<<<(unsigned)((I + 3) / 4), 256, 0, s>>>(I, R, ldr, V, p, M, A, Y, scale, C, kkt_tol, S, count);
 }
 
 void mse_grad(hipStream_t s, int B, const float *Y, const float *T, const unsigned char *rank, float inv_count,

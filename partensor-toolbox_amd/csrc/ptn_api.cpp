@@ -802,8 +802,20 @@ void update_nonneg(Work &w, int mode, const ptn_options &o, const ModeIn &in) {
     // the network proposes each row's start; rows failing the KKT test keep A (k_warm_select)
     mlp_features(s, I, w.R, w.ldr, in.V, w.M, A, nullptr, ms->X.p, nullptr, nullptr, ms->scale.p);
     ms->forward(s, (int)I);
-    warm_select(s, I, w.R, w.ldr, in.V, p, w.M, A, ms->Y.p, ms->scale.p, o.warm_kkt_tol, ms->start.p,
-                &p->warm_rows);
+    if (o.warm_refine > 0) {
+      // every finite proposal first takes warm_refine Nesterov steps of the same subproblem
+      // (anchor A, the global stop test included); the test then judges the refined rows
+      warm_select(s, I, w.R, w.ldr, in.V, p, w.M, A, ms->Y.p, ms->scale.p, nullptr, INFINITY, ms->start.p,
+                  nullptr);
+      nnls_iterate(s, I, w.R, w.ldr, in.V, p, in.betas, w.M, A, ms->start.p, o.nesterov_delta_1,
+                   o.nesterov_delta_2, o.warm_refine, w.mask, w.hist, w.colsum);
+      nnls_select(s, I, w.R, w.ldr, w.hist, w.mask, o.warm_refine, ms->start.p, w.colsum, p);
+      warm_select(s, I, w.R, w.ldr, in.V, p, w.M, A, nullptr, nullptr, ms->start.p, o.warm_kkt_tol, ms->start.p,
+                  &p->warm_rows);
+    } else {
+      warm_select(s, I, w.R, w.ldr, in.V, p, w.M, A, ms->Y.p, ms->scale.p, nullptr, o.warm_kkt_tol, ms->start.p,
+                  &p->warm_rows);
+    }
     start = ms->start.p;
   }
   // without a warm start the start point A_old stays in hist[0] for the fallback
@@ -875,6 +887,7 @@ void ptn_default_options(ptn_options *o) {
   o->fuse_accel = 0;
   o->warm_start = 0;
   o->warm_kkt_tol = 1e-6;
+  o->warm_refine = 0;
   o->mlp_harvest = 0;
 }
 
@@ -1076,6 +1089,8 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
             "fuse_accel needs the dimension tree with F32 or BF16X3 contractions");
     REQUIRE(!o.warm_start || (h->mlp && h->mlp->ready), PTN_ERR_STATE,
             "warm_start needs a network (ptn_mlp_init or ptn_mlp_set_params)");
+    REQUIRE(o.warm_refine >= 0 && o.warm_refine <= o.max_inner, PTN_ERR_INVALID,
+            "warm_refine must be in 0 .. max_inner");
 
     Work w(h, R, o.arith, tree);
     if (o.warm_start || o.mlp_harvest) {
@@ -1319,7 +1334,8 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
       lg.f_value = r.f_value;
       lg.rel_cost = r.rel_cost;
       for (int n = 0; n < N; ++n) {
-        lg.nnls_iters[n] = rb->hp[n].iters;
+        lg.nnls_iters[n] = rb->hp[n].iters + rb->hp[n].warm_steps;
+        lg.warm_steps[n] = rb->hp[n].warm_steps;
         lg.zero_col_fallback[n] = rb->hp[n].fallback;
         lg.lipschitz[n] = rb->hp[n].L0;
         lg.mu[n] = rb->hp[n].mu0;

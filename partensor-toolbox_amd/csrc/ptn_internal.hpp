@@ -33,7 +33,7 @@ struct NnlsParams {
   int iters;            // projected-gradient steps taken (k*)
   int fallback;         // zero-column blend applied
   int warm_rows;        // rows started from the warm-start network's proposal
-  int pad;
+  int warm_steps;       // Nesterov steps run from the network's proposals before the KKT test
 };
 
 // ---- batched small CPDs (batch_kernels.hip) ----
@@ -137,8 +137,10 @@ void gemm_nt(hipStream_t s, int M, int N, int K, const void *A, int lda, const v
 void transpose_bf16(hipStream_t s, int rows, int cols, const void *in, int ldi, void *out, int ldo);
 void mlp_features(hipStream_t s, int64_t I, int R, int ldr, const double *V, const double *M, const double *A,
                   const double *Anew, void *X, float *T, unsigned char *rank, float *scale);
-void warm_select(hipStream_t s, int64_t I, int R, int ldr, const double *V, const NnlsParams *p, const double *M,
-                 const double *A, const float *Y, const float *scale, double kkt_tol, double *S, int *count);
+// C: the proposals after warm_refine Nesterov steps (null: the network's outputs Y, scale)
+void warm_select(hipStream_t s, int64_t I, int R, int ldr, const double *V, NnlsParams *p, const double *M,
+                 const double *A, const float *Y, const float *scale, const double *C, double kkt_tol, double *S,
+                 int *count);
 void mse_grad(hipStream_t s, int B, const float *Y, const float *T, const unsigned char *rank, float inv_count,
               void *dYb, float *dY, double *part, int nparts);
 void colsum_f32(hipStream_t s, int M, int N, const float *G, int ldg, float *out);

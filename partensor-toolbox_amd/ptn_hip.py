@@ -48,7 +48,7 @@ class Options(ctypes.Structure):
         ("max_iter", _i), ("threshold_error", _d), ("nesterov_delta_1", _d), ("nesterov_delta_2", _d),
         ("accel_coeff", _i), ("accel_fail", _i), ("acceleration", _i), ("normalization", _i),
         ("constraints", _i * MAX_ORDER), ("mttkrp_form", _i), ("arith", _i), ("max_inner", _i),
-        ("fuse_accel", _i), ("warm_start", _i), ("warm_kkt_tol", _d), ("mlp_harvest", _i), ("time_passes", _i),
+        ("fuse_accel", _i), ("warm_start", _i), ("warm_kkt_tol", _d), ("warm_refine", _i), ("mlp_harvest", _i), ("time_passes", _i),
     ]
 
 
@@ -62,7 +62,7 @@ class SweepLog(ctypes.Structure):
         ("f_value", _d), ("rel_cost", _d), ("f_accel", _d), ("accel_tried", _i), ("accel_accepted", _i),
         ("nnls_iters", _i * MAX_ORDER), ("zero_col_fallback", _i * MAX_ORDER),
         ("lipschitz", _d * MAX_ORDER), ("mu", _d * MAX_ORDER), ("t_end", _d), ("warm_rows", _i * MAX_ORDER),
-        ("t_pass_ms", _d * 2),
+        ("t_pass_ms", _d * 2), ("warm_steps", _i * MAX_ORDER),
     ]
 
 
@@ -252,7 +252,8 @@ class Handle:
                         "accel_tried": r.accel_tried, "accel_accepted": r.accel_accepted,
                         "nnls_iters": list(r.nnls_iters[:N]), "zero_col_fallback": list(r.zero_col_fallback[:N]),
                         "lipschitz": list(r.lipschitz[:N]), "mu": list(r.mu[:N]), "t_end": r.t_end,
-                        "warm_rows": list(r.warm_rows[:N]), "t_pass_ms": list(r.t_pass_ms)})
+                        "warm_rows": list(r.warm_rows[:N]), "warm_steps": list(r.warm_steps[:N]),
+                        "t_pass_ms": list(r.t_pass_ms)})
         return unpack(out, dims, R), status, log
 
     def cpd_dimtree(self, R, init, **kw):

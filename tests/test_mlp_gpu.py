@@ -99,6 +99,19 @@ def test_rejected_proposals_leave_the_plain_result_bitwise(h):
         assert sw["f_value"] == sp["f_value"]
         for a, b in zip(Fw, Fp):
             assert np.array_equal(a, b)
+        # with refinement the non-finite proposals become the previous rows, which take the
+        # refinement steps and are then judged; at a gate of 0 none passes (a zero residual
+        # would need an exact solution), so the main solve is the plain one, and the counted
+        # steps are the plain ones plus the refinement's
+        Fr, sr, lr = h.cpd(R, init, max_iter=12, threshold_error=0.0, warm_start=1, warm_refine=5, warm_kkt_tol=0.0,
+                           mttkrp_form=form)
+        assert all(sum(l["warm_rows"]) == 0 for l in lr)
+        assert all(0 <= s <= 5 for l in lr for s in l["warm_steps"]) and sum(map(sum, (l["warm_steps"] for l in lr)))
+        assert [[a - b for a, b in zip(l["nnls_iters"], l["warm_steps"])] for l in lr] == \
+            [l["nnls_iters"] for l in lp]
+        assert sr["f_value"] == sp["f_value"]
+        for a, b in zip(Fr, Fp):
+            assert np.array_equal(a, b)
 
 
 def test_trained_network_on_a_held_out_problem(h):
@@ -141,6 +154,47 @@ def test_trained_network_on_a_held_out_problem(h):
                     assert np.array_equal(a, b)
         else:
             assert rows > 0 and np.isfinite(sw["rel_cost"])
+
+
+def test_warm_start_at_rank_64(h):
+    """The warm start at its configuration's rank on a scaled tensor: pairs harvested from
+    two 200^3 R = 64 runs (true rank 64, U[0,1) factors, SNR 15 dB), training, then a
+    held-out 200^3 R = 64 problem decomposed with the plain start, the network's start
+    under the 1e-6 KKT gate, the same after 10 refinement steps from every proposal, and
+    the refined proposals under a loose 5e-2 gate.  Reported: rows that took the network's
+    start, NNLS steps (refinement included) and the difference to the plain run, sweeps to
+    the plain run's final error.  Asserted (the gate's promise): under the 1e-6 gate, with
+    or without refinement, the final relative error is within 1e-6 of the plain run's."""
+    dims, R, sweeps = [200, 200, 200], 64, 20
+    h.mlp_data_clear()
+    for seed in (30, 32):
+        h.generate_tensor(dims, R, seed=seed, snr_db=15.0)
+        h.cpd(R, h.generate_factors(dims, R, seed=seed + 1), max_iter=sweeps, threshold_error=0.0, mlp_harvest=1)
+    pairs = h.mlp_data_size()
+    h.mlp_init(11)
+    loss = h.mlp_train(10, batch=512, lr=1e-3, seed=2)
+    say("R=64: %d pairs, training loss %.3e -> %.3e" % (pairs, loss[0], loss[-1]))
+    assert pairs == 2 * sweeps * sum(dims) and loss[-1] < loss[0]
+    h.generate_tensor(dims, R, seed=34, snr_db=15.0)
+    init = h.generate_factors(dims, R, seed=35)
+    kw = dict(max_iter=sweeps, threshold_error=0.0)
+    Fp, sp, lp = h.cpd(R, init, **kw)
+    ip = sum(sum(l["nnls_iters"]) for l in lp)
+    tgt = sp["rel_cost"] + 1e-9
+    kp = next(i + 1 for i, l in enumerate(lp) if l["rel_cost"] <= tgt)
+    for name, extra in (("gate 1e-6", dict(warm_kkt_tol=1e-6)),
+                        ("refine 10, gate 1e-6", dict(warm_kkt_tol=1e-6, warm_refine=10)),
+                        ("refine 10, gate 5e-2", dict(warm_kkt_tol=5e-2, warm_refine=10))):
+        Fw, sw, lw = h.cpd(R, init, warm_start=1, **extra, **kw)
+        iw = sum(sum(l["nnls_iters"]) for l in lw)
+        rows = sum(sum(l["warm_rows"]) for l in lw)
+        kw_ = next((i + 1 for i, l in enumerate(lw) if l["rel_cost"] <= tgt), None)
+        say("R=64 %s: rows from the network %d of %d; NNLS steps plain %d, warm %d (saved %d); sweeps to the "
+            "plain final error: plain %d, warm %s; final rel error plain %.12f warm %.12f"
+            % (name, rows, sweeps * sum(dims), ip, iw, ip - iw, kp, kw_, sp["rel_cost"], sw["rel_cost"]))
+        assert np.isfinite(sw["rel_cost"])
+        if extra["warm_kkt_tol"] == 1e-6:
+            assert abs(sw["rel_cost"] - sp["rel_cost"]) <= 1e-6
 
 
 def test_warm_start_needs_a_network():
