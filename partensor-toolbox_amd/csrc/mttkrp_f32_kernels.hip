@@ -25,6 +25,7 @@ namespace ptn {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x8 __attribute__((ext_vector_type(8)));
 
 namespace {
 
@@ -43,7 +44,9 @@ __device__ __forceinline__ void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
 //   c = 32 t + l%32 and element j holds W(k)[c] (0 for k >= K) with
 //   kperm 0:  k = 16 s16 + 8 (l/32) + j                       (lane pairs split 8 + 8)
 //   kperm 1:  k = 32 (s16/2) + 16 (l/32) + 8 (s16%2) + j       (lane pairs split 16 + 16)
-// W(k)[c] = prod_q fac_q[d_q(k)][c]; wf: FP32 row-major [k][32 NT] up to the padded K.
+// W(k)[c] = prod_q fac_q[d_q(k)][c]; wf: the same fragments in FP32 (one f32x8 per
+// (s16, t, lane), element j the FP32 value of the bf16 pair's W(k)), so that the FP32
+// contractions fetch a step's operand with one 32-byte load per column tile.
 // Column tiles t >= tsplit hold a second operand w2 (column c - 32 tsplit of its factors):
 // one contraction then serves two sets of factors.
 __global__ void k_pack_w(KrArgs32 w1, KrArgs32 w2, int tsplit, int64_t K, int64_t ksteps, int NT, int ldr,
@@ -60,6 +63,7 @@ __global__ void k_pack_w(KrArgs32 w1, KrArgs32 w2, int tsplit, int64_t K, int64_
     const int cf = second ? c - 32 * tsplit : c;
     const int64_t kbase = kperm ? 32 * (s16 / 2) + 16 * (l / 32) + 8 * (s16 % 2) : 16 * s16 + 8 * (l / 32);
     bf16x8 h, lo;
+    f32x8 fv;
     for (int j = 0; j < 8; ++j) {
       const int64_t k = kbase + j;
       double v = 0.0;
@@ -77,15 +81,16 @@ __global__ void k_pack_w(KrArgs32 w1, KrArgs32 w2, int tsplit, int64_t K, int64_
       split_bf16(f, a, b);
       h[j] = a;
       lo[j] = b;
-      if (wf) wf[k * (32 * NT) + c] = f;
+      fv[j] = f;
     }
     if (whi) { whi[e] = h; wlo[e] = lo; }
+    if (wf) ((f32x8 *)wf)[e] = fv;
   }
 }
 
 template <int MODE>
-__device__ __forceinline__ void mma_step(const float (&x)[8], const bf16x8 &bhi, const bf16x8 &blo,
-                                         const float *wf_rows, int RPF, int kg, int col, f32x16 &acc) {
+__device__ __forceinline__ void mma_step(const float (&x)[8], const bf16x8 &bhi, const bf16x8 &blo, const f32x8 &bf,
+                                         f32x16 &acc) {
   if constexpr (MODE == kGemmBf16x3) {
     bf16x8 ahi, alo;
 #pragma unroll
@@ -100,8 +105,7 @@ __device__ __forceinline__ void mma_step(const float (&x)[8], const bf16x8 &bhi,
     acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, bhi, acc, 0, 0, 0);
   } else {
 #pragma unroll
-    for (int q = 0; q < 8; ++q)
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(x[q], wf_rows[(8 * kg + q) * RPF + col], acc, 0, 0, 0);
+    for (int q = 0; q < 8; ++q) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(x[q], bf[q], acc, 0, 0, 0);
   }
 }
 
@@ -161,18 +165,20 @@ __global__ __launch_bounds__(256) void k_gemm_cm_f32(const float *__restrict__ X
   auto step = [&](int64_t k0, const float (&xc)[2][8]) {
     const int64_t ks = k0 >> 4;
     bf16x8 bh[NT], bl[NT];
-    if constexpr (MODE == kGemmBf16x3) {
+    f32x8 bf[NT];
 #pragma unroll
-      for (int t = 0; t < NT; ++t) {
+    for (int t = 0; t < NT; ++t) {
+      if constexpr (MODE == kGemmBf16x3) {
         bh[t] = whi[(ks * NT + t) * 64 + lane];
         bl[t] = wlo[(ks * NT + t) * 64 + lane];
+      } else {
+        bf[t] = ((const f32x8 *)wf)[(ks * NT + t) * 64 + lane];
       }
     }
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-      const float *wrow = wf ? wf + k0 * RPF : nullptr;
-      mma_step<MODE>(xc[0], bh[t], bl[t], wrow, RPF, kg, 32 * t + i, acc[0][t]);
-      mma_step<MODE>(xc[1], bh[t], bl[t], wrow, RPF, kg, 32 * t + i, acc[1][t]);
+      mma_step<MODE>(xc[0], bh[t], bl[t], bf[t], acc[0][t]);
+      mma_step<MODE>(xc[1], bh[t], bl[t], bf[t], acc[1][t]);
     }
   };
   int64_t k0 = kbeg;
@@ -307,15 +313,18 @@ __global__ __launch_bounds__(256) void k_gemm_ck_f32(const float *__restrict__ X
     // this step's packed operand first: memory counters retire in order, so loads
     // issued after the next step's tensor loads would make the MFMAs wait for them
     bf16x8 bh[2][NT], bl[2][NT];
-    if constexpr (MODE == kGemmBf16x3) {
+    f32x8 bf[2][NT];
 #pragma unroll
-      for (int s2 = 0; s2 < 2; ++s2)
+    for (int s2 = 0; s2 < 2; ++s2)
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
+      for (int t = 0; t < NT; ++t) {
+        if constexpr (MODE == kGemmBf16x3) {
           bh[s2][t] = whi[((s32 * 2 + s2) * NT + t) * 64 + lane];
           bl[s2][t] = wlo[((s32 * 2 + s2) * NT + t) * 64 + lane];
+        } else {
+          bf[s2][t] = ((const f32x8 *)wf)[((s32 * 2 + s2) * NT + t) * 64 + lane];
         }
-    }
+      }
     load(a0 + 32, xn);
 #pragma unroll
     for (int s2 = 0; s2 < 2; ++s2) {
@@ -325,9 +334,8 @@ __global__ __launch_bounds__(256) void k_gemm_ck_f32(const float *__restrict__ X
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
         // FP32 path: lane (c, kg) supplies W[a0 + 16 kg + 8 s2 + q][c] at MFMA step q
-        const float *wrow = wf ? wf + (a0 + 8 * s2) * RPF : nullptr;
-        mma_step<MODE>(x0, bh[s2][t], bl[s2][t], wrow, RPF, 2 * kg, 32 * t + i, acc[0][t]);
-        mma_step<MODE>(x1, bh[s2][t], bl[s2][t], wrow, RPF, 2 * kg, 32 * t + i, acc[1][t]);
+        mma_step<MODE>(x0, bh[s2][t], bl[s2][t], bf[s2][t], acc[0][t]);
+        mma_step<MODE>(x1, bh[s2][t], bl[s2][t], bf[s2][t], acc[1][t]);
       }
     }
   }
@@ -430,15 +438,18 @@ __global__ __launch_bounds__(256) void k_gemm_ck_lds(const float *__restrict__ X
     __syncthreads();
     const int64_t s32 = a0 >> 5;
     bf16x8 bh[2][NT], bl[2][NT];
-    if constexpr (MODE == kGemmBf16x3) {
+    f32x8 bf[2][NT];
 #pragma unroll
-      for (int s2 = 0; s2 < 2; ++s2)
+    for (int s2 = 0; s2 < 2; ++s2)
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
+      for (int t = 0; t < NT; ++t) {
+        if constexpr (MODE == kGemmBf16x3) {
           bh[s2][t] = whi[((s32 * 2 + s2) * NT + t) * 64 + lane];
           bl[s2][t] = wlo[((s32 * 2 + s2) * NT + t) * 64 + lane];
+        } else {
+          bf[s2][t] = ((const f32x8 *)wf)[((s32 * 2 + s2) * NT + t) * 64 + lane];
         }
-    }
+      }
     fetch(a0 + 32);
     float xc[2][16];
 #pragma unroll
@@ -458,9 +469,8 @@ __global__ __launch_bounds__(256) void k_gemm_ck_lds(const float *__restrict__ X
       for (int j = 0; j < 8; ++j) { x0[j] = xc[0][8 * s2 + j]; x1[j] = xc[1][8 * s2 + j]; }
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
-        const float *wrow = wf ? wf + (a0 + 8 * s2) * RPF : nullptr;
-        mma_step<MODE>(x0, bh[s2][t], bl[s2][t], wrow, RPF, 2 * kg, 32 * t + i, acc[0][t]);
-        mma_step<MODE>(x1, bh[s2][t], bl[s2][t], wrow, RPF, 2 * kg, 32 * t + i, acc[1][t]);
+        mma_step<MODE>(x0, bh[s2][t], bl[s2][t], bf[s2][t], acc[0][t]);
+        mma_step<MODE>(x1, bh[s2][t], bl[s2][t], bf[s2][t], acc[1][t]);
       }
     }
   }

@@ -383,7 +383,7 @@ struct Work {
       wlo.~DevBuf(); new (&wlo) DevBuf<uint16_t>();
       wf.~DevBuf(); new (&wf) DevBuf<float>();
       whi.alloc(h, frag * 8); wlo.alloc(h, frag * 8);
-      wf.alloc(h, (size_t)((K + 31) / 32 * 32) * width);
+      wf.alloc(h, frag * 8);
       wcap = frag;
     }
     const bool x3 = arith == PTN_ARITH_BF16X3;
