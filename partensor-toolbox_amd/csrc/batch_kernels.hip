@@ -468,25 +468,31 @@ __device__ __forceinline__ int nnls_block(int I, int R, BatchLds &S, double *Ag,
 #pragma unroll
     for (int tt = 0; tt < TPW; ++tt) {
       const int tile = wave + NW * tt;
-      if (tile < T) {
-        f64x4 g = {0.0, 0.0, 0.0, 0.0};
+      if (tile < T) {                                  // wave-uniform
+        // every LDS read first and unconditional (a row past I reads its own unused LDS row;
+        // its result is neither tested nor kept), then the MFMAs, then the rows
         const int ra = tile * 16 + m;                  // A fragment row (rows past I: not used)
+        double af[NQ], yv[4];
 #pragma unroll
-        for (int q = 0; q < NQ; ++q)
-          g = __builtin_amdgcn_mfma_f64_16x16x4f64(Y[(ra < kBMax ? ra : 0) * kBR + 4 * q + kk], zb[q], g, 0, 0, 0);
+        for (int q = 0; q < NQ; ++q) af[q] = Y[(ra < kBMax ? ra : 0) * kBR + 4 * q + kk];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) yv[j] = Y[(tile * 16 + kk + 4 * j) * kBR + m];
+        f64x4 g = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) g = __builtin_amdgcn_mfma_f64_16x16x4f64(af[q], zb[q], g, 0, 0, 0);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           const int row = tile * 16 + kk + 4 * j;
-          if (on && row < I) {
-            const double gg = g[j] - wt[tt][j];
-            const double y = Y[row * kBR + m];
-            if ((fabs(gg * y) > d1) || (gg < -d2)) pass = 0;
-            const double q0 = gg * invL;
-            const double tq = y - fma(fma(-q0, L, gg), invL, q0);
-            const double av = tq > 0.0 ? tq : 0.0;
-            an[tt][j] = av;
-            Yn[row * kBR + m] = (1.0 + beta) * av - beta * ac[tt][j];
-          }
+          const bool valid = on && row < I;
+          const double gg = g[j] - wt[tt][j];
+          const double y = yv[j];
+          if (valid && ((fabs(gg * y) > d1) || (gg < -d2))) pass = 0;
+          const double q0 = gg * invL;
+          const double tq = y - fma(fma(-q0, L, gg), invL, q0);
+          const double av = tq > 0.0 ? tq : 0.0;
+          if (valid) an[tt][j] = av;
+          const double yn = (1.0 + beta) * av - beta * ac[tt][j];
+          if (valid) Yn[row * kBR + m] = yn;
         }
       }
     }
