@@ -34,6 +34,7 @@ struct BatchLds {
   int colnz[kBR];
   int iters[3];
   int seg[2 * BatchArgs::kMaxThreads / 64];   // partial MTTKRP tiles per wave
+  uint64_t vote[2];           // Nesterov stop test: one byte per wave, two step parities
 };
 
 // threadIdx.x behind an empty asm statement: index arithmetic derived from it is redone
@@ -459,11 +460,18 @@ __device__ __forceinline__ int nnls_block(int I, int R, BatchLds &S, double *Ag,
   }
   if (t < kBR) S.colnz[t] = 0;
   __syncthreads();
+  // The stop test of step it (all rows, NesterovMNLS.hpp:217) is read one step later: step
+  // it + 1 is computed meanwhile from the same state, and if step it passed, its result (the
+  // iterate before step it, kept in ap) is returned and step it + 1 is discarded.  Each wave
+  // votes with one byte; the eight bytes are read as one word.
+  constexpr uint64_t kAll = NW >= 8 ? 0x0101010101010101ull : (1ull << (8 * NW)) / 255ull;
+  double ap[TPW][4];
   int it = 0, b = 0;
   for (;;) {
     const double *Y = b ? Yb1 : Yb0;
     double *Yn = b ? Yb0 : Yb1;
     const double beta = it < max_inner ? S.betas[it] : 0.0;
+    const uint64_t vprev = it > 0 ? S.vote[(it - 1) & 1] : 0ull;
     int pass = 1;
 #pragma unroll
     for (int tt = 0; tt < TPW; ++tt) {
@@ -496,12 +504,27 @@ __device__ __forceinline__ int nnls_block(int I, int R, BatchLds &S, double *Ag,
         }
       }
     }
-    const int all = __syncthreads_and(pass);
-    if (all || it >= max_inner) break;
+    {
+      const bool wall = __all(pass);
+      if (lane == 0) ((unsigned char *)&S.vote[it & 1])[wave] = wall ? 1 : 0;
+    }
+    __syncthreads();
+    if (it > 0 && (vprev & kAll) == kAll) {         // step it - 1 passed: its start is the result
+#pragma unroll
+      for (int tt = 0; tt < TPW; ++tt)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) ac[tt][j] = ap[tt][j];
+      --it;
+      break;
+    }
+    if (it >= max_inner) break;                     // the cap: the result is this step's start
 #pragma unroll
     for (int tt = 0; tt < TPW; ++tt)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) ac[tt][j] = an[tt][j];
+      for (int j = 0; j < 4; ++j) {
+        ap[tt][j] = ac[tt][j];
+        ac[tt][j] = an[tt][j];
+      }
     b ^= 1;
     ++it;
   }
