@@ -529,6 +529,21 @@ __global__ void k_reduce_splits_f32(const float *__restrict__ partials, int spli
     out[e] = s;
   }
 }
+// the same sums four at a time (n % 4 == 0): 16-byte loads of every split, in split order
+__global__ void k_reduce_splits_f32x4(const f32x4 *__restrict__ partials, int splits, int64_t n4,
+                                      f32x4 *__restrict__ out) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n4; e += (int64_t)gridDim.x * blockDim.x) {
+    f32x4 s = {0.f, 0.f, 0.f, 0.f};
+    for (int q = 0; q < splits; ++q) {
+      const f32x4 v = __builtin_nontemporal_load(partials + q * n4 + e);
+      s[0] += v[0];
+      s[1] += v[1];
+      s[2] += v[2];
+      s[3] += v[3];
+    }
+    out[e] = s;
+  }
+}
 
 KrArgs32 to_args32(const KrSpec &w) {
   KrArgs32 a{};
@@ -624,6 +639,12 @@ void reduce_chunks(hipStream_t s, const double *part, int64_t I2, int64_t wpf, i
 }
 
 void reduce_splits_f32(hipStream_t s, const float *partials, int splits, int64_t n, float *out) {
+  if (n % 4 == 0 && ((uintptr_t)partials | (uintptr_t)out) % 16 == 0) {
+    int64_t g = (n / 4 + 255) / 256;
+    if (g > 8192) g = 8192;
+    k_reduce_splits_f32x4<<<(unsigned)g, 256, 0, s>>>((const f32x4 *)partials, splits, n / 4, (f32x4 *)out);
+    return;
+  }
   int64_t g = (n + 255) / 256;
   if (g > 4096) g = 4096;
   k_reduce_splits_f32<<<(unsigned)g, 256, 0, s>>>(partials, splits, n, out);

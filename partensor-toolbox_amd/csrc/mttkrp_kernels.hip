@@ -498,6 +498,19 @@ __global__ void k_reduce_splits(const double *__restrict__ partials, int splits,
     out[e] = s;
   }
 }
+// the same sums two at a time (n % 2 == 0): 16-byte loads of every split, in split order
+__global__ void k_reduce_splits_x2(const f64x2 *__restrict__ partials, int splits, int64_t n2,
+                                   f64x2 *__restrict__ out) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n2; e += (int64_t)gridDim.x * blockDim.x) {
+    f64x2 s = {0.0, 0.0};
+    for (int q = 0; q < splits; ++q) {
+      const f64x2 v = partials[q * n2 + e];
+      s[0] += v[0];
+      s[1] += v[1];
+    }
+    out[e] = s;
+  }
+}
 
 }  // namespace
 
@@ -584,6 +597,12 @@ void contract2_pair(hipStream_t s, const float *Ya, const float *Yb, int64_t ld,
 }
 
 void reduce_splits(hipStream_t s, const double *partials, int splits, int64_t n, double *out) {
+  if (n % 2 == 0 && ((uintptr_t)partials | (uintptr_t)out) % 16 == 0) {
+    int64_t g = (n / 2 + 255) / 256;
+    if (g > 8192) g = 8192;
+    k_reduce_splits_x2<<<(unsigned)g, 256, 0, s>>>((const f64x2 *)partials, splits, n / 2, (f64x2 *)out);
+    return;
+  }
   int64_t g = (n + 255) / 256;
   if (g > 4096) g = 4096;
   k_reduce_splits<<<(unsigned)g, 256, 0, s>>>(partials, splits, n, out);
