@@ -134,7 +134,7 @@ __device__ __forceinline__ float load_nt(__amdgpu_buffer_rsrc_t r, uint32_t voff
 }
 
 template <int MODE, int NT, bool CAP, bool BUF>
-__global__ __launch_bounds__(256) void k_gemm_cm_f32(const float *__restrict__ X, int64_t M, int64_t K,
+__global__ __launch_bounds__(256, NT >= 4 ? 2 : 1) void k_gemm_cm_f32(const float *__restrict__ X, int64_t M, int64_t K,
                                                      int64_t kchunk, const bf16x8 *__restrict__ whi,
                                                      const bf16x8 *__restrict__ wlo, const float *__restrict__ wf,
                                                      float *__restrict__ out) {
@@ -390,7 +390,7 @@ __global__ __launch_bounds__(256) void k_gemm_ck_lds(const float *__restrict__ X
   constexpr int RPF = 32 * NT, TF = 256, LDP = 36;
   // unfused bf16x3: LDS padded to at most two blocks per CU (55 KB each), which measured
   // faster beside the solver kernels of the other streams; the fused epilogue is not
-  constexpr int PAD = (MODE == kGemmBf16x3 && !FUSE) ? 4608 : 0;
+  constexpr int PAD = (MODE == kGemmBf16x3 && !FUSE && NT == 1) ? 4608 : 0;
   __shared__ float tile[TF * LDP + PAD];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int i = lane & 31, kg = lane >> 5;
