@@ -1906,18 +1906,24 @@ ptn_status ptn_cpd_batch(ptn_handle h, const double *init_packed, const ptn_opti
       q.accel_coeff = o.accel_coeff;
       q.accel_fail = o.accel_fail;
     }
-    // Work of a problem: tensor elements times the padded rank.  A problem with more than
-    // kSplitWork times the batch's mean work has its MTTKRP sums cut in
-    // V = ceil(work / (kSplitWork mean)) pieces (at most 8), so that up to V blocks share it
-    // (BatchDesc::V); the pieces, and with them every result, depend on the batch only.  The
-    // other members repeat the solves, so splitting pays only for the largest problems: on the
-    // 512-problem batch (FP32 matrix cores, 512 x 256 launch) a factor of 4 gave 1.43 ms per
-    // sweep, 2.2 to 6 gave 1.41-1.46 ms, 1.5 gave 1.76 ms and no split 1.50 ms.
-    constexpr double kSplitWork = 4.0;
+    // Work of a problem: its tensor elements plus kRowWork per factor row.  One block's time on
+    // a problem (wall-clock stamps per problem over 100 sweeps of the 512-problem batch) is
+    // ~0.2 ns per element and MTTKRP pass whatever the rank (the matrix-core tile is 16 columns
+    // wide), plus ~0.35 us per factor row (Nesterov steps, Grams, Normalize): a rank-8
+    // 122 x 128 x 104 problem took as long as a rank-15 123 x 121 x 126 one cut in two, so the
+    // work is not weighted by the rank.  A problem with more than kSplitWork times the batch's
+    // mean work has its MTTKRP sums cut in V = ceil(work / (kSplitWork mean)) pieces (at most
+    // 8), so that up to V blocks share it (BatchDesc::V); the pieces, and with them every
+    // result, depend on the batch only.  The other members repeat the solves, so splitting
+    // pays only for the largest problems: on the 512-problem batch (512 x 256 launch) a factor
+    // of 3 gave 1.21 ms per sweep (FP32 matrix cores) and 1.29 ms (FP64 ones), against 1.39 and
+    // 1.55 ms with the rank-weighted work and a factor of 4.
+    constexpr double kSplitWork = 3.0;
+    constexpr double kRowWork = 540.0;
     std::vector<BatchDesc> dv(b.desc);
     auto work = [&](int p) {
       const BatchDesc &d = b.desc[p];
-      return (double)batch_elems(d) * ((d.R + 3) / 4 * 4 + 4);
+      return (double)batch_elems(d) + kRowWork * (d.I[0] + d.I[1] + d.I[2]);
     };
     double mean = 0.0;
     for (int p = 0; p < P; ++p) mean += work(p) / P;
