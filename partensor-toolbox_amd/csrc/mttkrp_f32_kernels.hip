@@ -115,11 +115,11 @@ __device__ __forceinline__ void mma_step(const float (&x)[8], const bf16x8 &bhi,
 // values of every lane (unconditional loads from clamped addresses), converts and
 // multiplies; occupancy hides the load latency.  The 64 x 32 result of each column tile
 // leaves through LDS as 16-byte non-temporal row stores: scattered 4-byte stores of the
-// MFMA layout cost several times their bytes next to the tensor stream.  CAP (bf16x3,
-// 32 or 64 columns): the staging area is padded so that at most two blocks share a CU;
-// the strided tensor stream runs faster with 8 waves per CU than with the 12-20 the
-// registers allow (800^3 left node, 32 columns: 3-10%; 200^4 order-4 node, 64 columns
-// split 13 ways: 1.113 ms against 1.234 ms).
+// MFMA layout cost several times their bytes next to the tensor stream.  CAP (bf16x3 at
+// 32 or 64 columns, FP32 products at 64): the staging area is padded so that at most two
+// blocks share a CU; the strided tensor stream runs faster with 8 waves per CU than with
+// the 12-20 the registers allow (800^3 left node, 32 columns: 3-10%; 200^4 order-4 node,
+// 64 columns split 13 ways: 1.113 ms against 1.234 ms bf16x3, 2.15 against 2.22 ms FP32).
 // ---------------------------------------------------------------------------
 // A raw buffer over `bytes` bytes from p (p uniform; its halves taken to scalar registers)
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_rsrc(const float *p, uint32_t bytes) {
@@ -134,14 +134,14 @@ __device__ __forceinline__ float load_nt(__amdgpu_buffer_rsrc_t r, uint32_t voff
 }
 
 template <int MODE, int NT, bool CAP, bool BUF>
-__global__ __launch_bounds__(256, NT >= 4 ? 2 : 1) void k_gemm_cm_f32(const float *__restrict__ X, int64_t M, int64_t K,
+__global__ __launch_bounds__(256, NT >= 4 || MODE == kGemmFp32 ? 2 : 1) void k_gemm_cm_f32(const float *__restrict__ X, int64_t M, int64_t K,
                                                      int64_t kchunk, const bf16x8 *__restrict__ whi,
                                                      const bf16x8 *__restrict__ wlo, const float *__restrict__ wf,
                                                      float *__restrict__ out) {
   constexpr int RPF = 32 * NT;
   constexpr int PAD = CAP ? 2048 : 0;                    // floats per wave: 66.5 KB per block
   __shared__ float stg[4][64 * 33 + PAD];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int i = lane & 31, kg = lane >> 5;
   const int64_t mb = ((int64_t)blockIdx.x * 4 + wave) * 64;
   if (mb >= M) return;                                   // no block-wide barrier below
@@ -189,15 +189,65 @@ __global__ __launch_bounds__(256, NT >= 4 ? 2 : 1) void k_gemm_cm_f32(const floa
     const __amdgpu_buffer_rsrc_t rx = raw_rsrc(X + kbeg * M, (uint32_t)((kend - kbeg) * M * 4));
     const uint32_t vo0 = (uint32_t)((row0 + 8 * kg * M) * 4), vo1 = (uint32_t)((row1 + 8 * kg * M) * 4);
     const uint32_t rowb = __builtin_amdgcn_readfirstlane((uint32_t)(M * 4));
-    for (; k0 + 16 <= kend; k0 += 16) {
-      const uint32_t sb = __builtin_amdgcn_readfirstlane((uint32_t)((k0 - kbeg) * M * 4));
-      float xc[2][8];
+    if constexpr (MODE == kGemmFp32 && NT <= 2) {
+      // FP32 products: 16 MFMAs per column tile and step leave the matrix cores the bound, so
+      // the next step's tensor values and operand fragments are loaded into a second set of
+      // registers while this step multiplies (same products, same order; 200^4 order-4 node
+      // at 64 columns: 2.42 -> 2.22 ms)
+      struct Buf {
+        float x[2][8];
+        f32x8 b[NT];
+      };
+      auto load = [&](int64_t k, Buf &u) {
+        const uint32_t sb = __builtin_amdgcn_readfirstlane((uint32_t)((k - kbeg) * M * 4));
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        xc[0][j] = load_nt(rx, vo0, sb + j * rowb);
-        xc[1][j] = load_nt(rx, vo1, sb + j * rowb);
+        for (int j = 0; j < 8; ++j) {
+          u.x[0][j] = load_nt(rx, vo0, sb + j * rowb);
+          u.x[1][j] = load_nt(rx, vo1, sb + j * rowb);
+        }
+#pragma unroll
+        for (int t = 0; t < NT; ++t) u.b[t] = ((const f32x8 *)wf)[((k >> 4) * NT + t) * 64 + lane];
+      };
+      auto mma = [&](const Buf &u) {
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          mma_step<MODE>(u.x[0], bf16x8{}, bf16x8{}, u.b[t], acc[0][t]);
+          mma_step<MODE>(u.x[1], bf16x8{}, bf16x8{}, u.b[t], acc[1][t]);
+        }
+      };
+      int n = __builtin_amdgcn_readfirstlane((int)((kend - k0) / 16));   // whole steps of the range
+      Buf b0, b1;
+      if (n & 1) {                                       // an odd first step alone
+        load(k0, b0);
+        mma(b0);
+        k0 += 16;
+        --n;
       }
-      step(k0, xc);
+      if (n > 0) {
+        // pairs of steps; the loads of the step after the last pair re-read the last step
+        // (unconditional, so the loop has no branch between a load and its use)
+        load(k0, b0);
+        for (int j = 0; j < n; j += 2) {
+          load(k0 + 16 * (j + 1), b1);
+          __builtin_amdgcn_sched_barrier(0);
+          mma(b0);
+          load(k0 + 16 * (j + 2 < n ? j + 2 : n - 1), b0);
+          __builtin_amdgcn_sched_barrier(0);
+          mma(b1);
+        }
+        k0 += 16 * n;
+      }
+    } else {
+      for (; k0 + 16 <= kend; k0 += 16) {
+        const uint32_t sb = __builtin_amdgcn_readfirstlane((uint32_t)((k0 - kbeg) * M * 4));
+        float xc[2][8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          xc[0][j] = load_nt(rx, vo0, sb + j * rowb);
+          xc[1][j] = load_nt(rx, vo1, sb + j * rowb);
+        }
+        step(k0, xc);
+      }
     }
   }
   for (; k0 < kend; k0 += 16) {
@@ -591,7 +641,7 @@ void gemm_cm_f32(hipStream_t s, GemmMode mode, const float *x32, int64_t M, int6
     else PTN_CM32(kGemmBf16x3, 4, false);
   } else {
     if (RPF == 32) PTN_CM32(kGemmFp32, 1, false);
-    else if (RPF == 64) PTN_CM32(kGemmFp32, 2, false);
+    else if (RPF == 64) PTN_CM32(kGemmFp32, 2, true);
     else PTN_CM32(kGemmFp32, 4, false);
   }
 #undef PTN_CM32

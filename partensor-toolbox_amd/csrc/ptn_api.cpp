@@ -410,11 +410,11 @@ int pick_splits(int64_t blocks_per_split, int64_t work_per_split_unit, int64_t m
   return (int)s;
 }
 
-// k splits of a strided contraction (gemm_cm) of Mrows rows over K.  bf16x3 launches run
-// at most two blocks per CU (512 on the chip), so the split count is rounded down to give
-// whole waves of 512 blocks: 200^4 at R = 64, 157 row blocks, 13 splits = 2041 blocks
-// (1.145 ms node with its reduction) against 14 = 2198 blocks (1.27 ms, a fifth wave
-// 30% full).
+// k splits of a strided contraction (gemm_cm) of Mrows rows over K.  FP32-stream launches run
+// at most two blocks per CU (512 on the chip; bf16x3 by its padded LDS, FP32 products by their
+// registers), so the split count is rounded down to give whole waves of 512 blocks: 200^4 at
+// R = 64, 157 row blocks, 13 splits = 2041 blocks (bf16x3: 1.145 ms node with its reduction)
+// against 14 = 2198 blocks (1.27 ms, a fifth wave 30% full).
 int pick_splits_cm(const Work &w, int64_t Mrows, int64_t K);
 
 KrSpec kr_of(Work &w, double *base, std::initializer_list<int> modes) {
@@ -442,7 +442,7 @@ KrSpec kr_range(Work &w, double *base, int lo, int hi) {
 // timed >= 0: the contraction kernel is bracketed by the events of pass `timed`
 int pick_splits_cm(const Work &w, int64_t Mrows, int64_t K) {
   const int64_t blocks = (Mrows + 255) / 256;
-  if (!(w.mfma32 && w.arith == PTN_ARITH_BF16X3)) return pick_splits(blocks, K / 256, 64);
+  if (!w.mfma32) return pick_splits(blocks, K / 256, 64);
   int64_t s = 2048 / blocks;
   if (s > 64) s = 64;
   if (s > K / 256) s = K / 256;
