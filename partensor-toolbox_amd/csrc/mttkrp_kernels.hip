@@ -215,25 +215,20 @@ __global__ __launch_bounds__(256, NT <= 2 ? 2 : 1) void k_gemm_ck_f64(const TX *
     xp[s] = X + L * (cv[s] ? c : 0);
   }
   // the Khatri-Rao row of the contracted modes at a (lowest mode fastest), NT fragments
+  // (the product runs over the digits as they are split off, so no array of row pointers
+  // is indexed at run time: such an array lives in scratch memory)
   auto kr_row = [&](int64_t a, bool av, double (&b)[NT]) {
-    const double *brow[kMaxOrder];
-    if (av) {
-      uint32_t rem = (uint32_t)a;                 // a < L: the last digit is what remains
-      for (int j = 0; j + 1 < w.nm; ++j) {
-        uint32_t d = rem % w.dim[j];
-        rem /= w.dim[j];
-        brow[j] = w.fac[j] + (int64_t)d * ldr;
-      }
-      brow[w.nm - 1] = w.fac[w.nm - 1] + (int64_t)rem * ldr;
-    }
 #pragma unroll
-    for (int u = 0; u < NT; ++u) {
-      double p = 0.0;
-      if (av) {
-        p = brow[0][16 * u + f];
-        for (int j = 1; j < w.nm; ++j) p *= brow[j][16 * u + f];
-      }
-      b[u] = p;
+    for (int u = 0; u < NT; ++u) b[u] = 0.0;
+    if (!av) return;
+    uint32_t rem = (uint32_t)a;                   // a < L: the last digit is what remains
+    for (int j = 0; j < w.nm; ++j) {
+      const bool lastd = j + 1 == w.nm;
+      const uint32_t d = lastd ? rem : rem % w.dim[j];
+      rem = lastd ? 0u : rem / w.dim[j];
+      const double *row = w.fac[j] + (int64_t)d * ldr;
+#pragma unroll
+      for (int u = 0; u < NT; ++u) b[u] = j == 0 ? row[16 * u + f] : b[u] * row[16 * u + f];
     }
   };
   auto mma16 = [&](const double (&x)[4][4], const double (&b)[4][NT]) {
