@@ -113,7 +113,9 @@ __device__ __forceinline__ void mma_step(const float (&x)[8], const bf16x8 &bhi,
 // gemm_cm_f32: each wave owns 64 consecutive rows (2 tiles of 32) and the block's
 // k-range.  Each 16-wide k step loads its operand fragments and the 2 x 8 tensor
 // values of every lane (unconditional loads from clamped addresses), converts and
-// multiplies; occupancy hides the load latency.  The 64 x 32 result of each column tile
+// multiplies; occupancy hides the load latency, except for FP32 products at up to 64
+// columns, which issue the next step's loads before the current step's MFMAs (two
+// register sets).  The 64 x 32 result of each column tile
 // leaves through LDS as 16-byte non-temporal row stores: scattered 4-byte stores of the
 // MFMA layout cost several times their bytes next to the tensor stream.  CAP (bf16x3 at
 // 32 or 64 columns, FP32 products at 64): the staging area is padded so that at most two
