@@ -124,7 +124,9 @@ __global__ __launch_bounds__(256, 2) void k_gemm_cm_f64(const TX *__restrict__ X
   // Groups of KU steps, software-pipelined over two register buffers: group G+1's loads
   // are issued before group G's MFMAs, with no branch between a load and its use inside
   // the loop.  The MFMAs run in step order (the same sums as one step at a time).
-  constexpr int KU = NT == 1 ? 4 : NT == 2 ? 2 : 1;
+  // (FP32 data at two column tiles: groups of one step, since two steps' converted values
+  // overflowed the registers into scratch)
+  constexpr int KU = NT == 1 ? 4 : NT == 2 && sizeof(TX) == 8 ? 2 : 1;
   struct Buf {
     bool kv[KU];
     double a[KU][4], p[KU][NT], q[KU][NT];
