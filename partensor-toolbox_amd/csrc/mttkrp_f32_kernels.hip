@@ -279,7 +279,12 @@ __global__ __launch_bounds__(256, NT >= 4 || MODE == kGemmFp32 ? 2 : 1) void k_g
       const int r = e >> 3, c4 = 4 * (e & 7);
       const int64_t row = mb + r;
       const f32x4 val = {st[r * 33 + c4], st[r * 33 + c4 + 1], st[r * 33 + c4 + 2], st[r * 33 + c4 + 3]};
-      if (row < M) __builtin_nontemporal_store(val, (f32x4 *)(o + row * RPF + 32 * t + c4));
+      if (row < M) {
+        // a node goes out non-temporally; split partials stay cached for their reduction
+        f32x4 *dst = (f32x4 *)(o + row * RPF + 32 * t + c4);
+        if (gridDim.y > 1) *dst = val;
+        else __builtin_nontemporal_store(val, dst);
+      }
     }
     __builtin_amdgcn_wave_barrier();
   }
