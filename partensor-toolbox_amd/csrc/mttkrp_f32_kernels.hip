@@ -135,7 +135,7 @@ __device__ __forceinline__ float load_nt(__amdgpu_buffer_rsrc_t r, uint32_t voff
   return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 2));   // 2: nt
 }
 
-template <int MODE, int NT, bool CAP, bool BUF>
+template <int MODE, int NT, bool CAP, bool BUF, bool KEEP>
 __global__ __launch_bounds__(256, NT >= 4 || MODE == kGemmFp32 ? 2 : 1) void k_gemm_cm_f32(const float *__restrict__ X, int64_t M, int64_t K,
                                                      int64_t kchunk, const bf16x8 *__restrict__ whi,
                                                      const bf16x8 *__restrict__ wlo, const float *__restrict__ wf,
@@ -280,9 +280,10 @@ __global__ __launch_bounds__(256, NT >= 4 || MODE == kGemmFp32 ? 2 : 1) void k_g
       const int64_t row = mb + r;
       const f32x4 val = {st[r * 33 + c4], st[r * 33 + c4 + 1], st[r * 33 + c4 + 2], st[r * 33 + c4 + 3]};
       if (row < M) {
-        // a node goes out non-temporally; split partials stay cached for their reduction
+        // KEEP (split partial sums): cached stores, read back soon by the split reduction;
+        // a whole node goes out non-temporally
         f32x4 *dst = (f32x4 *)(o + row * RPF + 32 * t + c4);
-        if (gridDim.y > 1) *dst = val;
+        if constexpr (KEEP) *dst = val;
         else __builtin_nontemporal_store(val, dst);
       }
     }
@@ -637,10 +638,15 @@ void gemm_cm_f32(hipStream_t s, GemmMode mode, const float *x32, int64_t M, int6
   const bf16x8 *h = (const bf16x8 *)whi, *l = (const bf16x8 *)wlo;
   // raw-buffer loads when a block's k range of the tensor spans less than 4 GB
   const bool buf = (kchunk + 16) * M * 4 < ((int64_t)1 << 32);
-#define PTN_CM32(MODEV, NTV, CAPV)                                                                 \
-  do {                                                                                            \
-    if (buf) k_gemm_cm_f32<MODEV, NTV, CAPV, true><<<grid, 256, 0, s>>>(x32, M, K, kchunk, h, l, wf, out); \
-    else k_gemm_cm_f32<MODEV, NTV, CAPV, false><<<grid, 256, 0, s>>>(x32, M, K, kchunk, h, l, wf, out); \
+#define PTN_CM32K(MODEV, NTV, CAPV, KEEPV)                                                                  \
+  do {                                                                                                      \
+    if (buf) k_gemm_cm_f32<MODEV, NTV, CAPV, true, KEEPV><<<grid, 256, 0, s>>>(x32, M, K, kchunk, h, l, wf, out); \
+    else k_gemm_cm_f32<MODEV, NTV, CAPV, false, KEEPV><<<grid, 256, 0, s>>>(x32, M, K, kchunk, h, l, wf, out); \
+  } while (0)
+#define PTN_CM32(MODEV, NTV, CAPV)              \
+  do {                                          \
+    if (splits > 1) PTN_CM32K(MODEV, NTV, CAPV, true); \
+    else PTN_CM32K(MODEV, NTV, CAPV, false);   \
   } while (0)
   if (mode == kGemmBf16x3) {
     if (RPF == 32) PTN_CM32(kGemmBf16x3, 1, true);
@@ -652,6 +658,7 @@ void gemm_cm_f32(hipStream_t s, GemmMode mode, const float *x32, int64_t M, int6
     else PTN_CM32(kGemmFp32, 4, false);
   }
 #undef PTN_CM32
+#undef PTN_CM32K
 }
 
 void gemm_ck_f32(hipStream_t s, GemmMode mode, const float *x32, int64_t L, int64_t C, int RPF, const void *whi,
