@@ -295,6 +295,8 @@ __global__ __launch_bounds__(256, NT <= 2 ? 2 : 1) void k_gemm_ck_f64(const TX *
     }
     mma16(x, b);
   }
+  // the node leaves with non-temporal stores (a leaf reads it after the other streams' work;
+  // 800^3 R = 32 right node 1.020 -> 0.993 ms)
 #pragma unroll
   for (int s = 0; s < 4; ++s)
 #pragma unroll
@@ -302,7 +304,7 @@ __global__ __launch_bounds__(256, NT <= 2 ? 2 : 1) void k_gemm_ck_f64(const TX *
       int64_t c = cb + 16 * s + g + 4 * v;
       if (c < C) {
 #pragma unroll
-        for (int u = 0; u < NT; ++u) T[c * ldr + 16 * u + f] = acc[s][u][v];
+        for (int u = 0; u < NT; ++u) __builtin_nontemporal_store(acc[s][u][v], &T[c * ldr + 16 * u + f]);
       }
     }
 }
