@@ -1190,17 +1190,20 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
       // decision, so it runs on the side stream beside the next pass (Normalize leaves
       // the last factor and its Gram as they are unless it is the weight factor)
       const bool side_cost = step_now && pair && !(o.normalization && !all_orth && wf == last);
+      const bool early_right = tree && w.mfma32 && N == 3;
       for (int n = 0; n < N; ++n) {
         if (!(n == 0 && prepped0)) {
           sel[n] = slot(w, n);
           fork();
           prep_mode(w, w.grams, n, tree, sel[n], o.max_inner, ss);
         }
+        // mode 1's set-up is queued on the side stream before the right node on the third,
+        // so that its eigen wave starts before the right node's blocks fill the CUs
+        if (n == 1 && early_right) right_node_early(w, w.fac);
         if (tree) mttkrp_tree(w, w.fac, n, w.M);
         else mttkrp_standard(w, w.fac, n, w.M);
         if (!(n == 0 && prepped0)) join();   // else joined after the line-search pass
         update_factor(w, n, o, sel[n]);
-        if (n == 0 && tree && w.mfma32 && N == 3) right_node_early(w, w.fac);
         if (n == last && !side_cost)
           cost_gram(ms, w.dims[last] * w.ldr, w.M, w.F(last), R, sel[last].V, w.G(last), w.scal.p, w.scal.p + 1, 0);
       }
