@@ -92,14 +92,15 @@ __global__ __launch_bounds__(256) void k_gram_combine(int R, int nchunks, const 
 // With ng > 0 the matrix is first formed as the Hadamard product of the ng Grams in
 // list order (V = G_l0, V = G_lj * V; PartialCwiseProd) and stored to Vout; with
 // betas != nullptr the momentum coefficients of the Nesterov solve follow the
-// eigenvalues (one kernel prepares an NNLS update).  One wave.
+// eigenvalues (one kernel prepares an NNLS update).  One block of four waves.
 template <int RB>
-__global__ __launch_bounds__(64) void k_eig_extremes(int R, const double *__restrict__ V, GramPtrs G, int ng,
-                                                     double *__restrict__ Vout, NnlsParams *prm, int max_inner,
-                                                     double *__restrict__ betas) {
+__global__ __launch_bounds__(256) void k_eig_extremes(int R, const double *__restrict__ V, GramPtrs G, int ng,
+                                                      double *__restrict__ Vout, NnlsParams *prm, int max_inner,
+                                                      double *__restrict__ betas) {
   __shared__ EigLds<RB> S;
-  const int n = R, l = threadIdx.x;
-  for (int x = l; x < RB * RB; x += 64) {
+  __shared__ double part[4][RB];
+  const int n = R;
+  for (int x = threadIdx.x; x < RB * RB; x += 256) {
     const int r = x / RB, c = x % RB;
     double v = 0.0;
     if (r < n && c < n) {
@@ -116,8 +117,8 @@ __global__ __launch_bounds__(64) void k_eig_extremes(int R, const double *__rest
   }
   __syncthreads();
   double amax, amin;
-  eig_extremes_wave<RB>(n, S, amax, amin);
-  if (l == 0) glambda_betas(amax, amin, prm, max_inner, betas);
+  eig_extremes_block<RB>(n, S, part, amax, amin);
+  if (threadIdx.x == 0) glambda_betas(amax, amin, prm, max_inner, betas);
 }
 
 
@@ -586,18 +587,18 @@ void gram(hipStream_t s, int64_t I, int R, int ldr, const double *A, double *G, 
 
 void eig_extremes(hipStream_t s, int R, const double *V, NnlsParams *p) {
   const GramPtrs G{};
-  if (R <= 16) k_eig_extremes<16><<<1, 64, 0, s>>>(R, V, G, 0, nullptr, p, 0, nullptr);
-  else if (R <= 32) k_eig_extremes<32><<<1, 64, 0, s>>>(R, V, G, 0, nullptr, p, 0, nullptr);
-  else k_eig_extremes<64><<<1, 64, 0, s>>>(R, V, G, 0, nullptr, p, 0, nullptr);
+  if (R <= 16) k_eig_extremes<16><<<1, 256, 0, s>>>(R, V, G, 0, nullptr, p, 0, nullptr);
+  else if (R <= 32) k_eig_extremes<32><<<1, 256, 0, s>>>(R, V, G, 0, nullptr, p, 0, nullptr);
+  else k_eig_extremes<64><<<1, 256, 0, s>>>(R, V, G, 0, nullptr, p, 0, nullptr);
 }
 
 void nnls_setup(hipStream_t s, int R, int ng, const double *const *grams, double *V, NnlsParams *p, int max_inner,
                 double *betas) {
   GramPtrs G{};
   for (int j = 0; j < ng; ++j) G.p[j] = grams[j];
-  if (R <= 16) k_eig_extremes<16><<<1, 64, 0, s>>>(R, nullptr, G, ng, V, p, max_inner, betas);
-  else if (R <= 32) k_eig_extremes<32><<<1, 64, 0, s>>>(R, nullptr, G, ng, V, p, max_inner, betas);
-  else k_eig_extremes<64><<<1, 64, 0, s>>>(R, nullptr, G, ng, V, p, max_inner, betas);
+  if (R <= 16) k_eig_extremes<16><<<1, 256, 0, s>>>(R, nullptr, G, ng, V, p, max_inner, betas);
+  else if (R <= 32) k_eig_extremes<32><<<1, 256, 0, s>>>(R, nullptr, G, ng, V, p, max_inner, betas);
+  else k_eig_extremes<64><<<1, 256, 0, s>>>(R, nullptr, G, ng, V, p, max_inner, betas);
 }
 
 void nnls_prep(hipStream_t s, const NnlsParams *p, int max_inner, double *betas) {
