@@ -64,19 +64,31 @@ __global__ void k_pack_w(KrArgs32 w1, KrArgs32 w2, int tsplit, int64_t K, int64_
     const int64_t kbase = kperm ? 32 * (s16 / 2) + 16 * (l / 32) + 8 * (s16 % 2) : 16 * s16 + 8 * (l / 32);
     bf16x8 h, lo;
     f32x8 fv;
+    // the eight k of the fragment mode by mode, so that each mode's eight row loads are
+    // independent and in flight together (same left-to-right products; an entry past K or
+    // past the rank reads row 0 / column 0 and is zeroed)
+    const bool cv = cf < ldr;
+    const int cc = cv ? cf : 0;
+    uint32_t rem[8];
+    double v[8];
+#pragma unroll
     for (int j = 0; j < 8; ++j) {
-      const int64_t k = kbase + j;
-      double v = 0.0;
-      if (k < K && cf < ldr) {
-        uint32_t rem = (uint32_t)k;
-        v = 1.0;
-        for (int q = 0; q < w.nm; ++q) {
-          uint32_t d = rem % w.dim[q];
-          rem /= w.dim[q];
-          v *= w.fac[q][(int64_t)d * ldr + cf];
-        }
+      rem[j] = kbase + j < K ? (uint32_t)(kbase + j) : 0u;
+      v[j] = 1.0;
+    }
+    for (int q = 0; q < w.nm; ++q) {
+      const double *fq = w.fac[q];
+      const uint32_t dq = w.dim[q];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const uint32_t d = rem[j] % dq;
+        rem[j] /= dq;
+        v[j] *= fq[(int64_t)d * ldr + cc];
       }
-      float f = (float)v;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      float f = (kbase + j < K && cv) ? (float)v[j] : 0.f;
       __bf16 a, b;
       split_bf16(f, a, b);
       h[j] = a;
