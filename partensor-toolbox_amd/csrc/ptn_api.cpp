@@ -627,11 +627,16 @@ void mttkrp_tree(Work &w, double *base, int mode, double *M) {
 // a sweep it is formed on the aux stream as soon as mode 0 is updated, beside mode 1's
 // contraction, eigen set-up and Nesterov solve; mode 2 then contracts it with A_1
 // (DimTrees.hpp:668-800: the same node, formed earlier).  The main stream waits for it
-// at mode 2.
-void right_node_early(Work &w, double *base) {
+// at mode 2.  `after`: an event the caller recorded on the main stream with nothing
+// enqueued there since (the aux stream then waits on it and the main queue takes no
+// second record); null records one here.
+void right_node_early(Work &w, double *base, hipEvent_t after = nullptr) {
   ptn_handle h = w.h;
-  HIPCHK(hipEventRecord(h->ev[2], h->stream));
-  HIPCHK(hipStreamWaitEvent(h->aux, h->ev[2], 0));
+  if (!after) {
+    HIPCHK(hipEventRecord(h->ev[2], h->stream));
+    after = h->ev[2];
+  }
+  HIPCHK(hipStreamWaitEvent(h->aux, after, 0));
   const Node nd = node_ck(w, kr_of(w, base, {0}), w.dims[0], w.dims[1] * w.dims[2], h->aux, 1);
   w.right.p64 = nd.p64; w.right.p32 = nd.p32; w.right.ld = nd.ld; w.right.sel = nullptr; w.right.sel_off = 0;
   HIPCHK(hipEventRecord(h->ev[3], h->aux));
@@ -1198,11 +1203,17 @@ ptn_status ptn_cpd(ptn_handle h, int R, const double *init_packed, const ptn_opt
           prep_mode(w, w.grams, n, tree, sel[n], o.max_inner, ss);
         }
         // mode 1's set-up is queued on the side stream before the right node on the third,
-        // so that its eigen wave starts before the right node's blocks fill the CUs
-        if (n == 1 && early_right) right_node_early(w, w.fac);
+        // so that its eigen wave starts before the right node's blocks fill the CUs; the
+        // third stream waits on the fork's event (only the side stream was fed since)
+        if (n == 1 && early_right) right_node_early(w, w.fac, h->ev[0]);
+        // mode 2's set-up ends on the side stream before the right node on the third, so
+        // the main stream joins the side stream before it waits for the right node: both
+        // waits then pass before the leaf contraction instead of one on each side of it
+        const bool join_early = n == 2 && early_right;
+        if (join_early) join();
         if (tree) mttkrp_tree(w, w.fac, n, w.M);
         else mttkrp_standard(w, w.fac, n, w.M);
-        if (!(n == 0 && prepped0)) join();   // else joined after the line-search pass
+        if (!(n == 0 && prepped0) && !join_early) join();   // mode 0: joined after the line-search pass
         update_factor(w, n, o, sel[n]);
         if (n == last && !side_cost)
           cost_gram(ms, w.dims[last] * w.ldr, w.M, w.F(last), R, sel[last].V, w.G(last), w.scal.p, w.scal.p + 1, 0);
