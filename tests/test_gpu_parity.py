@@ -169,6 +169,36 @@ def test_sym_extremes_match_oracle(h, golden_dir):
         assert L == Lo and mu == muo                       # same arithmetic sequence: bitwise
 
 
+@pytest.mark.parametrize("n,kind", [(1, "scalar"), (2, "pair"), (7, "indefinite"), (17, "indefinite"),
+                                    (33, "indefinite"), (64, "indefinite"), (20, "negdef"), (64, "psd_rankdef"),
+                                    (9, "diag"), (40, "blockdiag")])
+def test_sym_extremes_edge_cases_match_oracle(h, n, kind):
+    """Spectra the Nesterov constants rarely see: eigenvalues of both signs (the pair
+    bracketed around 0), all negative, rank-deficient, and matrices with zero columns to
+    eliminate (the no-reflection branch), at every padded size (16, 32, 64): bitwise."""
+    say("eig edge", n, kind)
+    rng = np.random.default_rng(100 + n)
+    if kind in ("scalar", "pair", "indefinite"):
+        B = rng.standard_normal((n, n))
+        V = 0.5 * (B + B.T)
+    elif kind == "negdef":
+        B = rng.random((n, 2 * n))
+        V = -(B @ B.T)
+    elif kind == "psd_rankdef":
+        B = rng.random((n, n // 2))
+        V = B @ B.T
+    elif kind == "diag":
+        V = np.diag(rng.standard_normal(n))
+    else:                                                  # two blocks: zero entries below the first
+        V = np.zeros((n, n))
+        for a, b in ((0, n // 2), (n // 2, n)):
+            B = rng.standard_normal((b - a, b - a))
+            V[a:b, a:b] = 0.5 * (B + B.T)
+    L, mu = h.sym_extremes(V)
+    Lo, muo, _ = o.sym_extremes(V)
+    assert L == Lo and mu == muo
+
+
 @pytest.mark.parametrize("I,R,d", [(7, 4, 1e-2), (300, 32, 1e-2), (129, 64, 1e-2), (50, 10, 0.0), (20, 5, 1e30)])
 def test_nesterov_matches_oracle(h, I, R, d):
     say("nnls", I, R, d)
