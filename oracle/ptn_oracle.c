@@ -460,8 +460,9 @@ static double tree_sum(double *x, int w)
    working matrix t is padded to RB = 16, 32 or 64 rows/columns of zeros; step k
    eliminates row k of t, forms p_l = sum_j t[j][l] v_j with four interleaved partial
    sums over all RB rows, and takes sigma and p.v as balanced tree sums over 64
-   entries.  This operation order is the GPU kernel's (one wave, lane l = index l),
-   so the two agree bit for bit; for n > 64 RB = n and the trees widen. */
+   entries.  This operation order is the GPU kernels' (lane l = index l; one wave, or
+   four waves that split the four partial sums and the rows of the rank-2 update), so
+   the two agree bit for bit; for n > 64 RB = n and the trees widen. */
 static void tridiagonalize(int n, const double *Vraw, double *d, double *e)
 {
   const int RB = n <= 16 ? 16 : n <= 32 ? 32 : n <= 64 ? 64 : n;
